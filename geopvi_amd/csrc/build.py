@@ -1,0 +1,73 @@
+"""Build libgeopvi_fwi2d.so for gfx950 with hipcc (in-tree, no JIT cache).
+
+    python -m geopvi_amd.csrc.build            # or geopvi_amd.csrc.build.build()
+
+Flags: -ffp-contract=off keeps every float operation separately rounded, as
+the reference's x86-64 build is; the packed-FP32 target feature is disabled
+(hipcc -O3 would otherwise form v_pk_* FP32 instructions).
+"""
+from __future__ import annotations
+
+import os
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+PKG = os.path.dirname(HERE)
+REPO = os.path.dirname(PKG)
+LIB = os.path.join(PKG, "libgeopvi_fwi2d.so")
+SOURCES = ["fwi2d_kernels.hip", "fwi2d_capi.hip"]
+HEADERS = ["fwi2d_device.h", "fwi2d_step.h", "fwi2d_kernels.h"]
+ARCH = os.environ.get("GPVI_OFFLOAD_ARCH", "gfx950")
+
+HIPCC_FLAGS = [
+    f"--offload-arch={ARCH}",
+    "-O3",
+    "-std=c++17",
+    "-fPIC",
+    "-ffp-contract=off",
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-Wall",
+    "-Wno-unused-command-line-argument",
+]
+
+
+def _hipcc() -> str:
+    for cand in (os.environ.get("HIPCC"), "/opt/rocm/bin/hipcc", "hipcc"):
+        if cand and (os.path.sep not in cand or os.path.exists(cand)):
+            return cand
+    return "hipcc"
+
+
+def build(verbose: bool = False) -> str:
+    """Compile every source and link the shared library; always rebuilds."""
+    objs = []
+    build_dir = os.path.join(PKG, "_build")
+    os.makedirs(build_dir, exist_ok=True)
+    inc = ["-I", os.path.join(REPO, "include"), "-I", HERE]
+    for src in SOURCES:
+        obj = os.path.join(build_dir, src.replace(".hip", ".o"))
+        cmd = [_hipcc(), *HIPCC_FLAGS, *inc, "-c", os.path.join(HERE, src), "-o", obj]
+        _run(cmd, verbose)
+        objs.append(obj)
+    tmp = LIB + ".tmp"
+    _run([_hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs], verbose)
+    os.replace(tmp, LIB)
+    return LIB
+
+
+def _run(cmd, verbose):
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    # hipcc's host pass warns that it does not know the packed-fp32 feature
+    noise = ("packed-fp32-ops", "not a recognized feature", "warning generated", "warnings generated")
+    err = "\n".join(l for l in res.stderr.splitlines() if not any(n in l for n in noise))
+    if res.returncode != 0:
+        raise RuntimeError(f"build failed: {' '.join(cmd)}\n{res.stderr}")
+    if verbose and err.strip():
+        print(err, file=sys.stderr)
+
+
+if __name__ == "__main__":
+    print(build(verbose=True))

@@ -1,0 +1,921 @@
+// C ABI of the FWI-2D solver (declared in include/geopvi_fwi2d.h).
+#include "../../include/geopvi_fwi2d.h"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <string>
+#include <tuple>
+#include <vector>
+
+#include "fwi2d_kernels.h"
+
+using namespace gpvi;
+
+namespace {
+
+thread_local std::string g_err;
+thread_local int g_compat_status = 0;
+
+int fail(int code, const char* fmt, ...)
+{
+    char buf[1024];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+
+#define HIP_TRY(expr)                                                                        \
+    do {                                                                                     \
+        hipError_t e_ = (expr);                                                              \
+        if (e_ != hipSuccess) return fail(GPVI_ERR_HIP, "%s: %s", #expr, hipGetErrorString(e_)); \
+    } while (0)
+
+struct Tuning {
+    int variant = 0;
+    int grid = 0;          // 0: automatic
+    int graph_steps = 32;  // steps per graph segment
+};
+Tuning g_tuning;
+std::mutex g_tuning_mu;
+
+Tuning tuning()
+{
+    std::lock_guard<std::mutex> lk(g_tuning_mu);
+    return g_tuning;
+}
+
+// Central second-derivative weights for solver half-width lc (configured
+// Lc + 1), the table of fdtd2D_modelling.c:1272-1326 as exact rationals.
+int fd_coeffs(int lc, float* fd)
+{
+    for (int i = 0; i < 8; ++i) fd[i] = 0.f;
+    switch (lc) {
+    case 4:
+        fd[0] = -49.0 / 18; fd[1] = 3.0 / 2; fd[2] = -3.0 / 20; fd[3] = 1.0 / 90;
+        return 0;
+    case 5:
+        fd[0] = -205.0 / 72; fd[1] = 8.0 / 5; fd[2] = -1.0 / 5; fd[3] = 8.0 / 315; fd[4] = -1.0 / 560;
+        return 0;
+    case 6:
+        fd[0] = -5269.0 / 1800; fd[1] = 5.0 / 3; fd[2] = -5.0 / 21; fd[3] = 5.0 / 126;
+        fd[4] = -5.0 / 1008; fd[5] = 1.0 / 3150;
+        return 0;
+    case 7:
+        fd[0] = -5369.0 / 1800; fd[1] = 12.0 / 7; fd[2] = -15.0 / 56; fd[3] = 10.0 / 189;
+        fd[4] = -1.0 / 112; fd[5] = 2.0 / 1925; fd[6] = -1.0 / 16632;
+        return 0;
+    default:
+        return -1;
+    }
+}
+
+int validate(const gpvi_fwi2d_params* p)
+{
+    if (!p) return fail(GPVI_ERR_INVALID, "params is NULL");
+    if (p->laplace_slover != 0)
+        return fail(GPVI_ERR_UNSUPPORTED, "laplace_slover=%d: only the finite-difference Laplacian (0) is built",
+                    p->laplace_slover);
+    if (p->Lc < 3 || p->Lc > 6) return fail(GPVI_ERR_UNSUPPORTED, "Lc=%d: supported 3..6", p->Lc);
+    if (p->nx < 1 || p->nz < 1 || p->ns < 1 || p->nt < 1 || p->nr < 1 || p->nt_interval < 1 || p->pml0 < 0)
+        return fail(GPVI_ERR_INVALID, "nx, nz, ns, nt, nr, nt_interval must be >= 1 and pml0 >= 0");
+    if (!(p->dx > 0) || !(p->dz > 0) || !(p->dt > 0) || !std::isfinite(p->f0) || p->f0 == 0.f)
+        return fail(GPVI_ERR_INVALID, "dx, dz, dt must be > 0 and f0 finite and non-zero");
+    if (p->nr > 1 && p->dr < 1) return fail(GPVI_ERR_INVALID, "dr=%d: receivers would coincide", p->dr);
+    const int lc = p->Lc + 1, pml = p->pml0 + lc;
+    const long ntx = (long)p->nx + 2 * pml, ntz = (long)p->nz + 2 * pml;
+    if (ntx > (1 << 20) || ntz > (1 << 20)) return fail(GPVI_ERR_INVALID, "grid too large");
+    const long s0 = (long)pml + p->ns0, s1 = s0 + (long)(p->ns - 1) * p->ds;
+    const long r0 = (long)pml + p->nr0, r1 = r0 + (long)(p->nr - 1) * p->dr;
+    const long siz = (long)pml + p->depths, riz = (long)pml + p->depthr;
+    if (s0 < 0 || s1 < 0 || s0 >= ntx || s1 >= ntx || siz < 0 || siz >= ntz)
+        return fail(GPVI_ERR_GEOMETRY, "a source lies outside the %ldx%ld padded grid", ntz, ntx);
+    if (r0 < 0 || r1 < 0 || r0 >= ntx || r1 >= ntx || riz < 0 || riz >= ntz)
+        return fail(GPVI_ERR_GEOMETRY, "a receiver lies outside the %ldx%ld padded grid", ntz, ntx);
+    return 0;
+}
+
+struct Geom {
+    int lc, pml, ntx, ntz, pitch, nx, nz, ni, nt, k, nr, ns, nsave;
+    int variant, tile_x, tile_z, tiles_x, tiles_z, tpu, threads;
+    int64_t fe;
+    float fd[8];
+};
+
+void make_geom(const gpvi_fwi2d_params* p, int variant, Geom* g)
+{
+    g->lc = p->Lc + 1;
+    g->pml = p->pml0 + g->lc;
+    g->ntx = p->nx + 2 * g->pml;
+    g->ntz = p->nz + 2 * g->pml;
+    g->pitch = (g->ntx + 31) / 32 * 32;
+    g->nx = p->nx; g->nz = p->nz; g->ni = p->nx * p->nz;
+    g->nt = p->nt; g->k = p->nt_interval; g->nr = p->nr; g->ns = p->ns;
+    g->nsave = (p->nt + p->nt_interval - 1) / p->nt_interval;
+    g->fe = (int64_t)g->pitch * g->ntz;
+    g->variant = variant;
+    variant_tile(variant, &g->tile_x, &g->tile_z);
+    g->threads = variant_threads(variant);
+    g->tiles_x = (g->ntx + g->tile_x - 1) / g->tile_x;
+    g->tiles_z = (g->ntz + g->tile_z - 1) / g->tile_z;
+    g->tpu = g->tiles_x * g->tiles_z;
+    fd_coeffs(g->lc, g->fd);
+}
+
+constexpr size_t kAlign = 256;
+
+struct Bump {
+    char* base;
+    size_t off = 0;
+    template <class T>
+    T* take(size_t count)
+    {
+        off = (off + kAlign - 1) / kAlign * kAlign;
+        T* p = base ? reinterpret_cast<T*>(base + off) : nullptr;
+        off += count * sizeof(T);
+        return p;
+    }
+};
+
+// Adjoint schedule: full store (nseg = 1) or checkpoint segments.
+struct Sched {
+    int nseg, ckpt, last_len, nslot;
+};
+
+Sched make_sched(const Geom& g, int ckpt_req)
+{
+    Sched s;
+    if (ckpt_req <= 0 || ckpt_req >= g.nt) {
+        s.nseg = 1; s.ckpt = g.nt; s.last_len = g.nt; s.nslot = g.nsave;
+        return s;
+    }
+    int c = (ckpt_req + g.k - 1) / g.k * g.k;
+    s.nseg = (g.nt + c - 1) / c;
+    if (s.nseg <= 1) {
+        s.nseg = 1; s.ckpt = g.nt; s.last_len = g.nt; s.nslot = g.nsave;
+        return s;
+    }
+    s.ckpt = c;
+    s.last_len = g.nt - (s.nseg - 1) * c;
+    s.nslot = c / g.k;
+    return s;
+}
+
+struct BatchLayout {
+    float *vel, *wav, *fwd, *adj, *ck, *save, *gpriv;
+    double* partial;
+    UnitDesc* units;
+    TileRef* tiles;
+    StepCtl* ctl;
+    int64_t fwd_stride, adj_stride, ck_stride, save_stride, grad_stride;
+    size_t bytes;
+};
+
+void batch_layout(const Geom& g, const Sched& sc, int nsample, bool tables, char* base, BatchLayout* L)
+{
+    Bump b{base};
+    const int64_t U = (int64_t)nsample * g.ns;
+    auto rnd = [](int64_t n) { return (n + 63) / 64 * 64; };  // keep per-unit blocks 256-B aligned
+    L->fwd_stride = rnd(3 * g.fe);
+    L->adj_stride = rnd(3 * g.fe);
+    L->ck_stride = sc.nseg > 1 ? rnd(2 * (int64_t)sc.nseg * g.fe) : 0;
+    L->save_stride = rnd((int64_t)sc.nslot * g.ni);
+    L->grad_stride = rnd(g.ni);
+    L->vel = b.take<float>((size_t)nsample * g.fe);
+    L->wav = b.take<float>(g.nt);
+    L->fwd = b.take<float>((size_t)(U * L->fwd_stride));
+    L->adj = b.take<float>((size_t)(U * L->adj_stride));
+    L->ck = sc.nseg > 1 ? b.take<float>((size_t)(U * L->ck_stride)) : nullptr;
+    L->save = b.take<float>((size_t)(U * L->save_stride));
+    L->gpriv = b.take<float>((size_t)(U * L->grad_stride));
+    L->partial = b.take<double>((size_t)nsample * misfit_blocks((int64_t)g.ns * g.nt * g.nr));
+    if (tables) {
+        L->units = b.take<UnitDesc>((size_t)U);
+        L->tiles = b.take<TileRef>((size_t)(U * g.tpu));
+        L->ctl = b.take<StepCtl>(1);
+    } else {
+        L->units = nullptr; L->tiles = nullptr; L->ctl = nullptr;
+    }
+    L->bytes = b.off;
+}
+
+int check_workspace(void* ws, size_t have, size_t need)
+{
+    if (!ws) return fail(GPVI_ERR_WORKSPACE, "workspace is NULL (%zu bytes needed)", need);
+    if (reinterpret_cast<uintptr_t>(ws) % kAlign)
+        return fail(GPVI_ERR_WORKSPACE, "workspace must be %zu-byte aligned", kAlign);
+    if (have < need) return fail(GPVI_ERR_WORKSPACE, "workspace has %zu bytes, %zu needed", have, need);
+    return 0;
+}
+
+JobDesc make_job(const gpvi_fwi2d_params* p, const Geom& g, const Sched& sc, int nsample, const BatchLayout& L,
+                 const float* obs, float* syn, int run_fwi, int phase, int store)
+{
+    JobDesc j;
+    std::memset(&j, 0, sizeof j);
+    j.nsample = nsample; j.ns = g.ns; j.tiles_x = g.tiles_x; j.tiles_z = g.tiles_z;
+    j.tile0 = 0; j.unit0 = 0; j.ntx = g.ntx; j.ntz = g.ntz;
+    j.pitch = g.pitch; j.nx = g.nx; j.nz = g.nz; j.pml = g.pml;
+    j.pml0 = p->pml0; j.lc = g.lc; j.nt = g.nt; j.k = g.k;
+    j.nr = g.nr; j.dr = (g.nr > 1) ? p->dr : 1; j.r_ix0 = g.pml + p->nr0; j.r_iz = g.pml + p->depthr;
+    j.s_ix0 = g.pml + p->ns0; j.ds = p->ds; j.s_iz = g.pml + p->depths; j.nseg = sc.nseg;
+    j.ckpt = sc.ckpt; j.last_len = sc.last_len; j.nslot = sc.nslot; j.start = 0;
+    const int back = (sc.nseg == 1) ? g.nt : 2 * g.nt;
+    if (phase == GPVI_PHASE_FORWARD) {
+        j.s_off = 0; j.total = g.nt;
+    } else if (phase == GPVI_PHASE_ADJOINT) {
+        j.s_off = g.nt; j.total = back;
+    } else {
+        j.s_off = 0; j.total = g.nt + back;
+    }
+    j.run_fwi = run_fwi ? 1 : 0;
+    j.store = store;
+    j.dt = p->dt; j.dx = p->dx; j.dz = p->dz;
+    for (int c = 0; c < 8; ++c) j.fd[c] = g.fd[c];
+    j.fe = g.fe;
+    j.fwd_stride = L.fwd_stride; j.adj_stride = L.adj_stride; j.ck_stride = L.ck_stride;
+    j.save_stride = L.save_stride; j.grad_stride = L.grad_stride;
+    j.fwd = L.fwd; j.adj = L.adj; j.ck = L.ck; j.save = L.save; j.gpriv = L.gpriv;
+    j.vel = L.vel; j.wav = L.wav; j.syn = syn; j.obs = obs;
+    return j;
+}
+
+int auto_grid(int ntiles)
+{
+    Tuning t = tuning();
+    if (t.grid > 0) return std::max(8, t.grid / 8 * 8);
+    int g = (ntiles + 7) / 8 * 8;
+    return std::max(8, std::min(g, 2048));
+}
+
+// ---------------------------------------------------------------------------
+// Graph segments: `steps` multi-unit step launches (one per half-width group
+// present) followed by the step-counter advance, built by capture on a
+// private stream.  Keyed by everything the captured launches bake in.
+struct GraphKey {
+    const void* units;
+    const void* tiles;
+    const void* ctl;
+    int grid, variant, lc_mask, steps, device;
+    bool operator<(const GraphKey& o) const
+    {
+        return std::tie(units, tiles, ctl, grid, variant, lc_mask, steps, device) <
+               std::tie(o.units, o.tiles, o.ctl, o.grid, o.variant, o.lc_mask, o.steps, o.device);
+    }
+};
+
+std::mutex g_graph_mu;
+std::map<GraphKey, hipGraphExec_t> g_graphs;
+std::map<int, hipStream_t> g_capture_streams;
+constexpr size_t kMaxGraphs = 64;
+
+int get_graph(const GraphKey& key, hipGraphExec_t* out)
+{
+    std::lock_guard<std::mutex> lk(g_graph_mu);
+    auto it = g_graphs.find(key);
+    if (it != g_graphs.end()) {
+        *out = it->second;
+        return 0;
+    }
+    if (g_graphs.size() >= kMaxGraphs) {
+        HIP_TRY(hipDeviceSynchronize());
+        for (auto& kv : g_graphs) (void)hipGraphExecDestroy(kv.second);
+        g_graphs.clear();
+    }
+    hipStream_t cap;
+    auto sit = g_capture_streams.find(key.device);
+    if (sit == g_capture_streams.end()) {
+        HIP_TRY(hipStreamCreateWithFlags(&cap, hipStreamNonBlocking));
+        g_capture_streams[key.device] = cap;
+    } else {
+        cap = sit->second;
+    }
+    HIP_TRY(hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal));
+    hipError_t err = hipSuccess;
+    for (int i = 0; i < key.steps && err == hipSuccess; ++i)
+        for (int lc = 4; lc <= 7 && err == hipSuccess; ++lc)
+            if (key.lc_mask & (1 << (lc - 4)))
+                err = launch_step_multi(key.variant, lc, key.grid, cap, (const UnitDesc*)key.units,
+                                        (const TileRef*)key.tiles, (const StepCtl*)key.ctl, i);
+    if (err == hipSuccess) err = launch_advance(cap, (StepCtl*)key.ctl, key.steps);
+    hipGraph_t graph = nullptr;
+    hipError_t e2 = hipStreamEndCapture(cap, &graph);
+    if (err != hipSuccess || e2 != hipSuccess) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return fail(GPVI_ERR_HIP, "graph capture failed: %s", hipGetErrorString(err != hipSuccess ? err : e2));
+    }
+    hipGraphExec_t exec;
+    hipError_t e3 = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (e3 != hipSuccess) return fail(GPVI_ERR_HIP, "hipGraphInstantiate: %s", hipGetErrorString(e3));
+    g_graphs[key] = exec;
+    *out = exec;
+    return 0;
+}
+
+int current_device()
+{
+    int d = 0;
+    (void)hipGetDevice(&d);
+    return d;
+}
+
+}  // namespace
+
+// ===========================================================================
+extern "C" {
+
+const char* gpvi_last_error(void) { return g_err.c_str(); }
+
+int gpvi_abi_version(void) { return GPVI_ABI_VERSION; }
+
+int gpvi_fwi2d_read_params(const char* path, gpvi_fwi2d_params* out)
+{
+    if (!path || !out) return fail(GPVI_ERR_INVALID, "NULL argument");
+    FILE* fp = std::fopen(path, "r");
+    if (!fp) return fail(GPVI_ERR_IO, "cannot open parameter file %s", path);
+    int* ints[15] = {&out->nx, &out->nz, &out->pml0, &out->Lc, &out->laplace_slover, &out->ns, &out->nt,
+                     &out->ds, &out->ns0, &out->depths, &out->depthr, &out->nr, &out->dr, &out->nr0,
+                     &out->nt_interval};
+    float* flts[4] = {&out->dx, &out->dz, &out->dt, &out->f0};
+    char line[256];
+    int rc = 0;
+    // Each value follows one comment line; whitespace after it is skipped.
+    for (int i = 0; i < 19 && !rc; ++i) {
+        if (!std::fgets(line, sizeof line, fp)) {
+            rc = fail(GPVI_ERR_IO, "%s: file ends before value %d of 19", path, i + 1);
+            break;
+        }
+        int got = (i < 15) ? std::fscanf(fp, " %d", ints[i]) : std::fscanf(fp, " %f", flts[i - 15]);
+        if (got != 1) rc = fail(GPVI_ERR_IO, "%s: value %d of 19 is not a number", path, i + 1);
+        else if (std::fscanf(fp, " ") < 0 && !std::feof(fp))
+            rc = fail(GPVI_ERR_IO, "%s: read error", path);
+    }
+    std::fclose(fp);
+    return rc;
+}
+
+int gpvi_fwi2d_validate(const gpvi_fwi2d_params* p) { return validate(p); }
+
+int gpvi_fwi2d_set_tuning(int variant, int grid, int graph_steps)
+{
+    if (variant < 0 || variant >= kNumVariants) return fail(GPVI_ERR_INVALID, "variant %d out of range", variant);
+    if (grid < 0 || graph_steps < 1 || graph_steps > 4096) return fail(GPVI_ERR_INVALID, "bad grid/graph_steps");
+    std::lock_guard<std::mutex> lk(g_tuning_mu);
+    g_tuning.variant = variant;
+    g_tuning.grid = grid;
+    g_tuning.graph_steps = graph_steps;
+    return 0;
+}
+
+int gpvi_fwi2d_get_tuning(int* variant, int* grid, int* graph_steps)
+{
+    Tuning t = tuning();
+    if (variant) *variant = t.variant;
+    if (grid) *grid = t.grid;
+    if (graph_steps) *graph_steps = t.graph_steps;
+    return 0;
+}
+
+int gpvi_fwi2d_tile_shape(int variant, int* tile_x, int* tile_z)
+{
+    if (variant < 0 || variant >= kNumVariants) return fail(GPVI_ERR_INVALID, "variant %d out of range", variant);
+    variant_tile(variant, tile_x, tile_z);
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// Plain solver
+
+static void plain_layout(const Geom& g, char* base, float** vel, float** wav, float** fwd, float** adj,
+                         float** save, size_t* bytes, bool store)
+{
+    Bump b{base};
+    *vel = b.take<float>(g.fe);
+    *wav = b.take<float>(g.nt);
+    *fwd = b.take<float>(3 * g.fe);
+    *adj = b.take<float>(3 * g.fe);
+    *save = store ? b.take<float>((size_t)g.nsave * g.ni) : nullptr;
+    *bytes = b.off;
+}
+
+size_t gpvi_fwi2d_plain_workspace_bytes(const gpvi_fwi2d_params* p)
+{
+    if (validate(p)) return 0;
+    Geom g;
+    make_geom(p, 0, &g);
+    float *a, *b, *c, *d, *e;
+    size_t bytes;
+    plain_layout(g, nullptr, &a, &b, &c, &d, &e, &bytes, true);
+    return bytes;
+}
+
+static UnitDesc plain_unit(const gpvi_fwi2d_params* p, const Geom& g)
+{
+    UnitDesc u;
+    std::memset(&u, 0, sizeof u);
+    u.ntx = g.ntx; u.ntz = g.ntz; u.pitch = g.pitch; u.nx = g.nx;
+    u.nz = g.nz; u.pml = g.pml; u.pml0 = p->pml0; u.lc = g.lc;
+    u.nt = g.nt; u.k = g.k; u.nr = g.nr; u.dr = (g.nr > 1) ? p->dr : 1;
+    u.r_ix0 = g.pml + p->nr0; u.r_iz = g.pml + p->depthr; u.s_iz = g.pml + p->depths;
+    u.nseg = 1; u.ckpt = g.nt; u.last_len = g.nt; u.nslot = g.nsave;
+    u.tiles_x = g.tiles_x; u.tiles_z = g.tiles_z;
+    u.dt = p->dt; u.dx = p->dx; u.dz = p->dz;
+    for (int c = 0; c < 8; ++c) u.fd[c] = g.fd[c];
+    u.field_elems = g.fe;
+    return u;
+}
+
+static int plain_run(const gpvi_fwi2d_params* p, const float* vel_inner, float* record_syn,
+                     const float* record_obs, float* grad, int run_fwi, bool adjoint, void* workspace,
+                     size_t workspace_bytes, void* stream)
+{
+    int rc = validate(p);
+    if (rc) return rc;
+    if (!vel_inner || !record_syn || (adjoint && (!record_obs || !grad)))
+        return fail(GPVI_ERR_INVALID, "NULL buffer");
+    Geom g;
+    make_geom(p, 0, &g);
+    float *vel, *wav, *fwd, *adj, *save;
+    size_t need;
+    plain_layout(g, nullptr, &vel, &wav, &fwd, &adj, &save, &need, adjoint);
+    if ((rc = check_workspace(workspace, workspace_bytes, need))) return rc;
+    plain_layout(g, (char*)workspace, &vel, &wav, &fwd, &adj, &save, &need, adjoint);
+    hipStream_t st = (hipStream_t)stream;
+    HIP_TRY(launch_pad_velocity(st, vel_inner, vel, g.nx, g.nz, g.pml, g.ntx, g.ntz, g.pitch, g.fe, 1));
+    HIP_TRY(launch_ricker(st, wav, g.nt, p->dt, p->f0));
+    UnitDesc u = plain_unit(p, g);
+    u.fwd = fwd; u.adj = adj; u.save = save; u.grad = grad; u.vel = vel; u.wav = wav;
+    u.run_fwi = run_fwi ? 1 : 0;
+    u.grad_init = 0;            // accumulate into the caller's gradient, shot after shot
+    u.store = adjoint ? 1 : 0;   // wavefield store
+    u.total = adjoint ? 2 * g.nt : g.nt;
+    const int64_t rec = (int64_t)g.nt * g.nr;
+    for (int is = 0; is < g.ns; ++is) {
+        u.s_ix = is * p->ds + g.pml + p->ns0;
+        u.syn = record_syn + is * rec;
+        u.obs = adjoint ? record_obs + is * rec : nullptr;
+        for (int s = 0; s < u.total; ++s) HIP_TRY(launch_step_single(g.variant, g.lc, g.tpu, st, u, s));
+    }
+    return 0;
+}
+
+int gpvi_fwi2d_fwi_device(const gpvi_fwi2d_params* p, const float* vel_inner, float* record_syn,
+                          const float* record_obs, float* grad, int run_fwi, void* workspace,
+                          size_t workspace_bytes, void* stream)
+{
+    return plain_run(p, vel_inner, record_syn, record_obs, grad, run_fwi, true, workspace, workspace_bytes,
+                     stream);
+}
+
+int gpvi_fwi2d_forward_device(const gpvi_fwi2d_params* p, const float* vel_inner, float* record_syn,
+                              void* workspace, size_t workspace_bytes, void* stream)
+{
+    return plain_run(p, vel_inner, record_syn, nullptr, nullptr, 1, false, workspace, workspace_bytes, stream);
+}
+
+// ---------------------------------------------------------------------------
+// Batched solver
+
+size_t gpvi_fwi2d_batch_workspace_bytes(const gpvi_fwi2d_params* p, int nsample, int ckpt_interval)
+{
+    if (validate(p) || nsample < 1) return 0;
+    Geom g;
+    make_geom(p, tuning().variant, &g);
+    Sched sc = make_sched(g, ckpt_interval);
+    BatchLayout L;
+    batch_layout(g, sc, nsample, true, nullptr, &L);
+    return L.bytes;
+}
+
+int gpvi_fwi2d_batch(const gpvi_fwi2d_params* p, int nsample, const float* vel_inner, const float* record_obs,
+                     float* record_syn, float* grad, double* misfit, int run_fwi, int ckpt_interval, int phase,
+                     int flags, void* workspace, size_t workspace_bytes, void* stream)
+{
+    int rc = validate(p);
+    if (rc) return rc;
+    if (nsample < 1) return fail(GPVI_ERR_INVALID, "nsample=%d", nsample);
+    if (phase < GPVI_PHASE_ALL || phase > GPVI_PHASE_ADJOINT) return fail(GPVI_ERR_INVALID, "phase=%d", phase);
+    if (!record_obs || !record_syn || (phase != GPVI_PHASE_ADJOINT && !vel_inner) ||
+        (phase != GPVI_PHASE_FORWARD && !grad))
+        return fail(GPVI_ERR_INVALID, "NULL buffer");
+    const Tuning tn = tuning();
+    Geom g;
+    make_geom(p, tn.variant, &g);
+    Sched sc = make_sched(g, ckpt_interval);
+    BatchLayout L;
+    batch_layout(g, sc, nsample, true, nullptr, &L);
+    if ((rc = check_workspace(workspace, workspace_bytes, L.bytes))) return rc;
+    batch_layout(g, sc, nsample, true, (char*)workspace, &L);
+    hipStream_t st = (hipStream_t)stream;
+    if (phase != GPVI_PHASE_ADJOINT) {
+        HIP_TRY(launch_pad_velocity(st, vel_inner, L.vel, g.nx, g.nz, g.pml, g.ntx, g.ntz, g.pitch, g.fe, nsample));
+        HIP_TRY(launch_ricker(st, L.wav, g.nt, p->dt, p->f0));
+    }
+    const int store = (phase == GPVI_PHASE_FORWARD && (flags & GPVI_FLAG_NO_STORE)) ? 0 : 1;
+    JobDesc j = make_job(p, g, sc, nsample, L, record_obs, record_syn, run_fwi, phase, store);
+    HIP_TRY(launch_build_units(st, j, L.units, L.tiles));
+    const int ntiles = nsample * g.ns * g.tpu;
+    int n4[4] = {0, 0, 0, 0}, o4[4] = {0, 0, 0, 0};
+    n4[g.lc - 4] = ntiles;
+    HIP_TRY(launch_ctl_set(st, L.ctl, 0, n4, o4));
+    const int grid = auto_grid(ntiles);
+    if (flags & GPVI_FLAG_NO_GRAPH) {
+        for (int s = 0; s < j.total; ++s)
+            HIP_TRY(launch_step_multi(g.variant, g.lc, grid, st, L.units, L.tiles, L.ctl, s));
+    } else {
+        const int seg = std::min(tn.graph_steps, j.total);
+        GraphKey key{L.units, L.tiles, L.ctl, grid, g.variant, 1 << (g.lc - 4), seg, current_device()};
+        hipGraphExec_t exec;
+        if ((rc = get_graph(key, &exec))) return rc;
+        const int reps = (j.total + seg - 1) / seg;
+        for (int r = 0; r < reps; ++r) HIP_TRY(hipGraphLaunch(exec, st));
+    }
+    if (phase != GPVI_PHASE_FORWARD)
+        HIP_TRY(launch_grad_reduce(st, L.gpriv, L.grad_stride, grad, g.ns, g.ni, nsample));
+    if (phase != GPVI_PHASE_ADJOINT && misfit)
+        HIP_TRY(launch_misfit(st, record_syn, record_obs, (int64_t)g.ns * g.nt * g.nr, nsample, L.partial, misfit));
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// Engine
+
+}  // extern "C"
+
+namespace {
+
+struct EngineJob {
+    long long id;
+    Geom g;
+    Sched sc;
+    JobDesc j;
+    BatchLayout L;
+    int nunit;
+    long long end_step;
+    float* grad;
+    double* misfit;
+    float* syn;
+    const float* obs;
+    hipEvent_t done;
+    bool cancelled;
+};
+
+}  // namespace
+
+struct gpvi_engine {
+    hipStream_t stream;
+    int device;
+    int max_units, max_tiles, grid, variant, seg_steps;
+    UnitDesc* units;
+    TileRef* tiles;
+    StepCtl* ctl;
+    long long gstep;
+    long long next_id;
+    bool dirty;
+    std::vector<EngineJob> active;
+    std::map<long long, hipEvent_t> finished;
+    std::vector<unsigned char> slot_used;
+    std::vector<hipEvent_t> inflight;
+};
+
+namespace {
+
+int engine_alloc_slots(gpvi_engine* e, int n)
+{
+    int run = 0;
+    for (int i = 0; i < e->max_units; ++i) {
+        run = e->slot_used[i] ? 0 : run + 1;
+        if (run == n) {
+            const int first = i - n + 1;
+            for (int k = first; k <= i; ++k) e->slot_used[k] = 1;
+            return first;
+        }
+    }
+    return -1;
+}
+
+void engine_free_slots(gpvi_engine* e, int first, int n)
+{
+    for (int k = first; k < first + n; ++k) e->slot_used[k] = 0;
+}
+
+int engine_rebuild(gpvi_engine* e)
+{
+    int n4[4] = {0, 0, 0, 0}, o4[4] = {0, 0, 0, 0};
+    for (const auto& jb : e->active) n4[jb.g.lc - 4] += jb.nunit * jb.g.tpu;
+    int off = 0;
+    for (int g = 0; g < 4; ++g) {
+        o4[g] = off;
+        off += n4[g];
+    }
+    int cur[4] = {o4[0], o4[1], o4[2], o4[3]};
+    for (auto& jb : e->active) {
+        jb.j.tile0 = cur[jb.g.lc - 4];
+        cur[jb.g.lc - 4] += jb.nunit * jb.g.tpu;
+        HIP_TRY(launch_build_units(e->stream, jb.j, e->units, e->tiles));
+    }
+    HIP_TRY(launch_ctl_set(e->stream, e->ctl, (int)e->gstep, n4, o4));
+    e->dirty = false;
+    return 0;
+}
+
+int engine_finalize(gpvi_engine* e, EngineJob& jb)
+{
+    if (!jb.cancelled) {
+        HIP_TRY(launch_grad_reduce(e->stream, jb.L.gpriv, jb.L.grad_stride, jb.grad, jb.g.ns, jb.g.ni,
+                                   jb.j.nsample));
+        if (jb.misfit)
+            HIP_TRY(launch_misfit(e->stream, jb.syn, jb.obs, (int64_t)jb.g.ns * jb.g.nt * jb.g.nr, jb.j.nsample,
+                                  jb.L.partial, jb.misfit));
+    }
+    HIP_TRY(hipEventRecord(jb.done, e->stream));
+    return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int gpvi_engine_create(void* stream, int max_units, int max_tiles, gpvi_engine** out)
+{
+    if (!out || max_units < 1 || max_tiles < 1) return fail(GPVI_ERR_INVALID, "bad engine arguments");
+    gpvi_engine* e = new gpvi_engine();
+    e->stream = (hipStream_t)stream;
+    e->device = current_device();
+    const Tuning t = tuning();
+    e->max_units = max_units;
+    e->max_tiles = max_tiles;
+    e->variant = t.variant;
+    e->seg_steps = t.graph_steps;
+    e->grid = t.grid > 0 ? std::max(8, t.grid / 8 * 8) : 2048;
+    e->gstep = 0;
+    e->next_id = 1;
+    e->dirty = true;
+    e->slot_used.assign(max_units, 0);
+    hipError_t err = hipMalloc((void**)&e->units, sizeof(UnitDesc) * (size_t)max_units);
+    if (err == hipSuccess) err = hipMalloc((void**)&e->tiles, sizeof(TileRef) * (size_t)max_tiles);
+    if (err == hipSuccess) err = hipMalloc((void**)&e->ctl, sizeof(StepCtl));
+    if (err != hipSuccess) {
+        delete e;
+        return fail(GPVI_ERR_HIP, "engine tables: %s", hipGetErrorString(err));
+    }
+    *out = e;
+    return 0;
+}
+
+int gpvi_engine_sync(gpvi_engine* e)
+{
+    if (!e) return fail(GPVI_ERR_INVALID, "NULL engine");
+    HIP_TRY(hipStreamSynchronize(e->stream));
+    for (auto ev : e->inflight) (void)hipEventDestroy(ev);
+    e->inflight.clear();
+    return 0;
+}
+
+int gpvi_engine_destroy(gpvi_engine* e)
+{
+    if (!e) return 0;
+    (void)hipStreamSynchronize(e->stream);
+    for (auto ev : e->inflight) (void)hipEventDestroy(ev);
+    for (auto& jb : e->active) (void)hipEventDestroy(jb.done);
+    for (auto& kv : e->finished) (void)hipEventDestroy(kv.second);
+    {
+        std::lock_guard<std::mutex> lk(g_graph_mu);
+        for (auto it = g_graphs.begin(); it != g_graphs.end();) {
+            if (it->first.ctl == e->ctl) {
+                (void)hipGraphExecDestroy(it->second);
+                it = g_graphs.erase(it);
+            } else {
+                ++it;
+            }
+        }
+    }
+    (void)hipFree(e->units);
+    (void)hipFree(e->tiles);
+    (void)hipFree(e->ctl);
+    delete e;
+    return 0;
+}
+
+int gpvi_engine_submit(gpvi_engine* e, const gpvi_fwi2d_params* p, int nsample, const float* vel_inner,
+                       const float* record_obs, float* record_syn, float* grad, double* misfit, int run_fwi,
+                       int ckpt_interval, void* workspace, size_t workspace_bytes, long long* job_id)
+{
+    if (!e || !job_id) return fail(GPVI_ERR_INVALID, "NULL engine or job_id");
+    int rc = validate(p);
+    if (rc) return rc;
+    if (nsample < 1 || !vel_inner || !record_obs || !record_syn || !grad)
+        return fail(GPVI_ERR_INVALID, "bad job arguments");
+    EngineJob jb;
+    make_geom(p, e->variant, &jb.g);
+    jb.sc = make_sched(jb.g, ckpt_interval);
+    batch_layout(jb.g, jb.sc, nsample, false, nullptr, &jb.L);
+    if ((rc = check_workspace(workspace, workspace_bytes, jb.L.bytes))) return rc;
+    batch_layout(jb.g, jb.sc, nsample, false, (char*)workspace, &jb.L);
+    jb.nunit = nsample * jb.g.ns;
+    int tiles_now = 0;
+    for (const auto& a : e->active) tiles_now += a.nunit * a.g.tpu;
+    if (tiles_now + jb.nunit * jb.g.tpu > e->max_tiles)
+        return fail(GPVI_ERR_CAPACITY, "tile list full (%d + %d > %d)", tiles_now, jb.nunit * jb.g.tpu, e->max_tiles);
+    const int first = engine_alloc_slots(e, jb.nunit);
+    if (first < 0) return fail(GPVI_ERR_CAPACITY, "no run of %d free unit slots", jb.nunit);
+    hipError_t err = hipEventCreateWithFlags(&jb.done, hipEventDisableTiming);
+    if (err != hipSuccess) {
+        engine_free_slots(e, first, jb.nunit);
+        return fail(GPVI_ERR_HIP, "hipEventCreate: %s", hipGetErrorString(err));
+    }
+    err = launch_pad_velocity(e->stream, vel_inner, jb.L.vel, jb.g.nx, jb.g.nz, jb.g.pml, jb.g.ntx, jb.g.ntz,
+                              jb.g.pitch, jb.g.fe, nsample);
+    if (err == hipSuccess) err = launch_ricker(e->stream, jb.L.wav, jb.g.nt, p->dt, p->f0);
+    if (err != hipSuccess) {
+        engine_free_slots(e, first, jb.nunit);
+        (void)hipEventDestroy(jb.done);
+        return fail(GPVI_ERR_HIP, "job setup: %s", hipGetErrorString(err));
+    }
+    jb.j = make_job(p, jb.g, jb.sc, nsample, jb.L, record_obs, record_syn, run_fwi, GPVI_PHASE_ALL, 1);
+    jb.j.unit0 = first;
+    jb.j.start = (int)e->gstep;
+    jb.end_step = e->gstep + jb.j.total;
+    jb.grad = grad;
+    jb.misfit = misfit;
+    jb.syn = record_syn;
+    jb.obs = record_obs;
+    jb.cancelled = false;
+    jb.id = e->next_id++;
+    e->active.push_back(jb);
+    e->dirty = true;
+    *job_id = jb.id;
+    return 0;
+}
+
+int gpvi_engine_advance(gpvi_engine* e, int nsteps)
+{
+    if (!e) return fail(GPVI_ERR_INVALID, "NULL engine");
+    int rc;
+    if (e->dirty && (rc = engine_rebuild(e))) return rc;
+    if (e->active.empty() || nsteps <= 0) return 0;
+    int mask = 0;
+    for (const auto& jb : e->active) mask |= 1 << (jb.g.lc - 4);
+    GraphKey key{e->units, e->tiles, e->ctl, e->grid, e->variant, mask, e->seg_steps, e->device};
+    hipGraphExec_t exec;
+    if ((rc = get_graph(key, &exec))) return rc;
+    const int reps = (nsteps + e->seg_steps - 1) / e->seg_steps;
+    for (int r = 0; r < reps; ++r) {
+        // Keep at most three segments queued ahead of the device.
+        while (e->inflight.size() >= 3) {
+            HIP_TRY(hipEventSynchronize(e->inflight.front()));
+            (void)hipEventDestroy(e->inflight.front());
+            e->inflight.erase(e->inflight.begin());
+        }
+        HIP_TRY(hipGraphLaunch(exec, e->stream));
+        hipEvent_t ev;
+        HIP_TRY(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+        HIP_TRY(hipEventRecord(ev, e->stream));
+        e->inflight.push_back(ev);
+        e->gstep += e->seg_steps;
+    }
+    int ncomplete = 0;
+    for (size_t i = 0; i < e->active.size();) {
+        EngineJob& jb = e->active[i];
+        if (jb.end_step <= e->gstep) {
+            if ((rc = engine_finalize(e, jb))) return rc;
+            engine_free_slots(e, jb.j.unit0, jb.nunit);
+            e->finished[jb.id] = jb.done;
+            e->active.erase(e->active.begin() + i);
+            e->dirty = true;
+            ++ncomplete;
+        } else {
+            ++i;
+        }
+    }
+    return ncomplete;
+}
+
+int gpvi_engine_job_status(gpvi_engine* e, long long job_id)
+{
+    if (!e) return fail(GPVI_ERR_INVALID, "NULL engine");
+    auto it = e->finished.find(job_id);
+    if (it != e->finished.end()) {
+        hipError_t q = hipEventQuery(it->second);
+        if (q == hipSuccess) return 1;
+        if (q == hipErrorNotReady) return 0;
+        return fail(GPVI_ERR_HIP, "hipEventQuery: %s", hipGetErrorString(q));
+    }
+    for (const auto& jb : e->active)
+        if (jb.id == job_id) return 0;
+    return fail(GPVI_ERR_UNKNOWN_JOB, "unknown job %lld", job_id);
+}
+
+int gpvi_engine_release(gpvi_engine* e, long long job_id)
+{
+    if (!e) return fail(GPVI_ERR_INVALID, "NULL engine");
+    auto it = e->finished.find(job_id);
+    if (it == e->finished.end()) return fail(GPVI_ERR_UNKNOWN_JOB, "job %lld is not finished", job_id);
+    HIP_TRY(hipEventSynchronize(it->second));
+    (void)hipEventDestroy(it->second);
+    e->finished.erase(it);
+    return 0;
+}
+
+int gpvi_engine_cancel(gpvi_engine* e, long long job_id)
+{
+    if (!e) return fail(GPVI_ERR_INVALID, "NULL engine");
+    for (size_t i = 0; i < e->active.size(); ++i) {
+        EngineJob& jb = e->active[i];
+        if (jb.id != job_id) continue;
+        jb.cancelled = true;
+        int rc = engine_finalize(e, jb);  // records the event after all work already queued
+        if (rc) return rc;
+        engine_free_slots(e, jb.j.unit0, jb.nunit);
+        e->finished[jb.id] = jb.done;
+        e->active.erase(e->active.begin() + i);
+        e->dirty = true;
+        return 0;
+    }
+    return fail(GPVI_ERR_UNKNOWN_JOB, "job %lld is not active", job_id);
+}
+
+int gpvi_engine_active_jobs(gpvi_engine* e) { return e ? (int)e->active.size() : 0; }
+
+int gpvi_engine_active_units(gpvi_engine* e)
+{
+    if (!e) return 0;
+    int n = 0;
+    for (const auto& jb : e->active) n += jb.nunit;
+    return n;
+}
+
+int gpvi_engine_pending_steps(gpvi_engine* e)
+{
+    if (!e) return 0;
+    long long m = 0;
+    for (const auto& jb : e->active) m = std::max(m, jb.end_step - e->gstep);
+    return (int)m;
+}
+
+// ---------------------------------------------------------------------------
+// Reference-compatible host entry points
+
+static int compat_run(char input_file[200], float* vel_inner, float* record_syn, float* record_obs, float* grad,
+                      int run_fwi, bool adjoint, int verbose)
+{
+    gpvi_fwi2d_params p;
+    int rc = gpvi_fwi2d_read_params(input_file, &p);
+    if (rc) return rc;
+    if ((rc = validate(&p))) return rc;
+    if (verbose)
+        std::printf("fwi2d: nx=%d nz=%d nt=%d ns=%d dx=%f dz=%f dt=%f\n", p.nx, p.nz, p.nt, p.ns, p.dx, p.dz, p.dt);
+    const size_t ni = (size_t)p.nx * p.nz, nrec = (size_t)p.ns * p.nt * p.nr;
+    const size_t ws = gpvi_fwi2d_plain_workspace_bytes(&p);
+    float *d_vel = nullptr, *d_syn = nullptr, *d_obs = nullptr, *d_grad = nullptr;
+    void* d_ws = nullptr;
+    hipError_t err = hipMalloc((void**)&d_vel, ni * sizeof(float));
+    if (err == hipSuccess) err = hipMalloc((void**)&d_syn, nrec * sizeof(float));
+    if (err == hipSuccess && adjoint) err = hipMalloc((void**)&d_obs, nrec * sizeof(float));
+    if (err == hipSuccess && adjoint) err = hipMalloc((void**)&d_grad, ni * sizeof(float));
+    if (err == hipSuccess) err = hipMalloc(&d_ws, ws);
+    if (err == hipSuccess) err = hipMemcpy(d_vel, vel_inner, ni * sizeof(float), hipMemcpyHostToDevice);
+    if (err == hipSuccess && adjoint) err = hipMemcpy(d_obs, record_obs, nrec * sizeof(float), hipMemcpyHostToDevice);
+    if (err == hipSuccess && adjoint) err = hipMemcpy(d_grad, grad, ni * sizeof(float), hipMemcpyHostToDevice);
+    if (err != hipSuccess) rc = fail(GPVI_ERR_HIP, "fwi_2D setup: %s", hipGetErrorString(err));
+    if (!rc) {
+        rc = adjoint ? gpvi_fwi2d_fwi_device(&p, d_vel, d_syn, d_obs, d_grad, run_fwi, d_ws, ws, nullptr)
+                     : gpvi_fwi2d_forward_device(&p, d_vel, d_syn, d_ws, ws, nullptr);
+    }
+    if (!rc) {
+        err = hipDeviceSynchronize();
+        if (err == hipSuccess) err = hipMemcpy(record_syn, d_syn, nrec * sizeof(float), hipMemcpyDeviceToHost);
+        if (err == hipSuccess && adjoint) err = hipMemcpy(grad, d_grad, ni * sizeof(float), hipMemcpyDeviceToHost);
+        if (err != hipSuccess) rc = fail(GPVI_ERR_HIP, "fwi_2D: %s", hipGetErrorString(err));
+    }
+    (void)hipFree(d_vel); (void)hipFree(d_syn); (void)hipFree(d_obs); (void)hipFree(d_grad); (void)hipFree(d_ws);
+    return rc;
+}
+
+void fwi_2D(char input_file[200], float* vel_inner, float* record_syn, float* record_obs, float* grad, int run_fwi,
+            int verbose)
+{
+    g_compat_status = compat_run(input_file, vel_inner, record_syn, record_obs, grad, run_fwi, true, verbose);
+    if (g_compat_status) std::fprintf(stderr, "fwi_2D: %s\n", g_err.c_str());
+}
+
+void forward_2D(char input_file[200], float* vel_inner, float* record_syn, int run_fwi, int verbose)
+{
+    (void)run_fwi;
+    g_compat_status = compat_run(input_file, vel_inner, record_syn, nullptr, nullptr, 1, false, verbose);
+    if (g_compat_status) std::fprintf(stderr, "forward_2D: %s\n", g_err.c_str());
+}
+
+int gpvi_compat_last_status(void) { return g_compat_status; }
+
+}  // extern "C"

@@ -1,0 +1,59 @@
+// Host-side view of the FWI-2D kernels (launchers and job descriptors).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <algorithm>
+
+#include "fwi2d_device.h"
+
+namespace gpvi {
+
+// Everything needed to build the unit table of one job (nsample samples x
+// ns shots) on the device; passed by value as a kernel argument so no host
+// buffer has to outlive an asynchronous copy.
+struct JobDesc {
+    int nsample, ns, tiles_x, tiles_z;
+    int tile0, unit0, ntx, ntz;
+    int pitch, nx, nz, pml;
+    int pml0, lc, nt, k;
+    int nr, dr, r_ix0, r_iz;
+    int s_ix0, ds, s_iz, nseg;
+    int ckpt, last_len, nslot, start;
+    int s_off, total, run_fwi, store;
+    float dt, dx, dz, pad1;
+    float fd[8];
+    int64_t fe;
+    int64_t fwd_stride, adj_stride, ck_stride, save_stride, grad_stride;
+    float* fwd;
+    float* adj;
+    float* ck;
+    float* save;
+    float* gpriv;
+    const float* vel;
+    const float* wav;
+    float* syn;
+    const float* obs;
+};
+
+void variant_tile(int variant, int* tx, int* tz);
+int variant_threads(int variant);
+constexpr int kNumVariants = 4;
+
+hipError_t launch_step_single(int variant, int lc, int grid, hipStream_t st, const UnitDesc& u, int s);
+hipError_t launch_step_multi(int variant, int lc, int grid, hipStream_t st, const UnitDesc* units,
+                             const TileRef* tiles, const StepCtl* ctl, int step_off);
+hipError_t launch_advance(hipStream_t st, StepCtl* ctl, int n);
+hipError_t launch_ctl_set(hipStream_t st, StepCtl* ctl, int step, const int* ntiles4, const int* off4);
+hipError_t launch_pad_velocity(hipStream_t st, const float* vin, float* vpad, int nx, int nz, int pml,
+                               int ntx, int ntz, int pitch, int64_t fe, int nsample);
+hipError_t launch_ricker(hipStream_t st, float* wav, int nt, float dt, float f0);
+hipError_t launch_build_units(hipStream_t st, const JobDesc& j, UnitDesc* units, TileRef* tiles);
+hipError_t launch_grad_reduce(hipStream_t st, const float* gpriv, int64_t grad_stride, float* out, int ns,
+                              int ni, int nsample);
+int misfit_blocks(int64_t len);
+hipError_t launch_misfit(hipStream_t st, const float* syn, const float* obs, int64_t len, int nsample,
+                         double* partial, double* misfit);
+
+}  // namespace gpvi

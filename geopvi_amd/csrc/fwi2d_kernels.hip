@@ -1,0 +1,344 @@
+// HIP kernels of the FWI-2D solver for gfx950: the fused step (single-unit
+// and multi-unit persistent forms), setup and reduction kernels.
+#include "fwi2d_kernels.h"
+#include "fwi2d_step.h"
+
+namespace gpvi {
+
+// ---------------------------------------------------------------------------
+// Step kernels
+
+// One unit, one tile per workgroup (the plain per-shot solver).
+template <int LC, int TX, int TZ, int NTH>
+__global__ __launch_bounds__(NTH) void step_single_kernel(UnitDesc u, int s)
+{
+    __shared__ float s_p1[TileCfg<LC, TX, TZ, NTH>::kLds];
+    process_tile<LC, TX, TZ, NTH>(u, s, blockIdx.x, s_p1);
+}
+
+// Many units per launch.  A fixed grid walks the tile list; tiles are
+// split into 8 contiguous chunks, one per XCD (workgroups are dispatched to
+// XCDs round-robin), so halo neighbours of a tile tend to be read through
+// the same L2.  Step and tile count come from device memory so that a
+// captured graph can be replayed without re-instantiation.
+template <int LC, int TX, int TZ, int NTH>
+__global__ __launch_bounds__(NTH) void step_multi_kernel(const UnitDesc* __restrict__ units,
+                                                         const TileRef* __restrict__ tiles,
+                                                         const StepCtl* __restrict__ ctl, int step_off)
+{
+    __shared__ float s_p1[TileCfg<LC, TX, TZ, NTH>::kLds];
+    const int lcs = LC - 4;
+    const int ntiles = ctl->ntiles_lc[lcs];
+    const int toff = ctl->tile_off[lcs];
+    const int gstep = ctl->step + step_off;
+    const int G = gridDim.x;
+    const int b = blockIdx.x;
+    const int xcd = b & 7, lb = b >> 3, gper = G >> 3;
+    const int chunk = (ntiles + 7) >> 3;
+    const int t_begin = xcd * chunk;
+    const int t_end = min(ntiles, t_begin + chunk);
+    for (int t = t_begin + lb; t < t_end; t += gper) {
+        const TileRef tr = tiles[toff + t];
+        const UnitDesc& u = units[tr.unit];
+        const int s = gstep - u.start + u.s_off;
+        if (s < u.s_off || s >= u.s_off + u.total) continue;
+        process_tile<LC, TX, TZ, NTH>(u, s, tr.tile, s_p1);
+        __syncthreads();
+    }
+}
+
+__global__ void advance_kernel(StepCtl* ctl, int n)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) ctl->step = ctl->step + n;
+}
+
+__global__ void ctl_set_kernel(StepCtl* ctl, int step, int n4, int n5, int n6, int n7, int o4, int o5,
+                               int o6, int o7)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        ctl->step = step;
+        ctl->ntiles_lc[0] = n4; ctl->ntiles_lc[1] = n5; ctl->ntiles_lc[2] = n6; ctl->ntiles_lc[3] = n7;
+        ctl->tile_off[0] = o4; ctl->tile_off[1] = o5; ctl->tile_off[2] = o6; ctl->tile_off[3] = o7;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Setup kernels
+
+// Edge-replicated padding of nsample inner models (fdtd2D_modelling.c:1187-1238):
+// padded(iz, ix) = inner(clamp(iz - pml), clamp(ix - pml)).
+__global__ void pad_velocity_kernel(const float* __restrict__ vin, float* __restrict__ vpad, int nx,
+                                    int nz, int pml, int ntx, int ntz, int pitch, int64_t fe, int nsample)
+{
+    const int64_t per = (int64_t)ntz * pitch;
+    const int64_t total = per * nsample;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const int m = (int)(i / per);
+        const int64_t r = i - (int64_t)m * per;
+        const int iz = (int)(r / pitch), ix = (int)(r - (int64_t)iz * pitch);
+        float val = 0.f;
+        if (ix < ntx) {
+            const int jz = min(max(iz - pml, 0), nz - 1);
+            const int jx = min(max(ix - pml, 0), nx - 1);
+            val = vin[(int64_t)m * nx * nz + (int64_t)jz * nx + jx];
+        }
+        vpad[(int64_t)m * fe + r] = val;
+    }
+}
+
+// Ricker wavelet (fdtd2D_modelling.c:1084-1093): the squared argument is
+// rounded to float, the wavelet is formed in double and rounded to float.
+__global__ void ricker_kernel(float* __restrict__ wav, int nt, float dt, float f0)
+{
+    const int it = blockIdx.x * blockDim.x + threadIdx.x;
+    if (it >= nt) return;
+    const float arg = (float)it * dt - 1.f / f0;
+    const double x = 3.1415926 * (double)f0 * (double)arg;
+    const float tau = (float)(x * x);
+    wav[it] = (float)((1.0 - 2.0 * (double)tau) * exp(-(double)tau));
+}
+
+// Fill the unit table and tile list of one job from its descriptor.
+__global__ void build_units_kernel(JobDesc j, UnitDesc* __restrict__ units, TileRef* __restrict__ tiles)
+{
+    const int nunit = j.nsample * j.ns;
+    const int tpu = j.tiles_x * j.tiles_z;
+    const int64_t ntile = (int64_t)nunit * tpu;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < ntile;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const int un = (int)(i / tpu);
+        tiles[j.tile0 + i] = TileRef{j.unit0 + un, (int)(i - (int64_t)un * tpu)};
+    }
+    const int un = blockIdx.x * blockDim.x + threadIdx.x;
+    if (un >= nunit) return;
+    const int m = un / j.ns, sh = un - (un / j.ns) * j.ns;
+    UnitDesc u;
+    u.ntx = j.ntx; u.ntz = j.ntz; u.pitch = j.pitch; u.nx = j.nx;
+    u.nz = j.nz; u.pml = j.pml; u.pml0 = j.pml0; u.lc = j.lc;
+    u.nt = j.nt; u.k = j.k; u.nr = j.nr; u.dr = j.dr;
+    u.r_ix0 = j.r_ix0; u.r_iz = j.r_iz; u.s_ix = j.s_ix0 + sh * j.ds; u.s_iz = j.s_iz;
+    u.nseg = j.nseg; u.ckpt = j.ckpt; u.last_len = j.last_len; u.nslot = j.nslot;
+    u.tiles_x = j.tiles_x; u.tiles_z = j.tiles_z; u.start = j.start; u.s_off = j.s_off;
+    u.total = j.total; u.run_fwi = j.run_fwi; u.grad_init = 1; u.store = j.store;
+    u.dt = j.dt; u.dx = j.dx; u.dz = j.dz; u.pad1 = 0.f;
+    for (int c = 0; c < 8; ++c) u.fd[c] = j.fd[c];
+    u.field_elems = j.fe;
+    u.fwd = j.fwd + (int64_t)un * j.fwd_stride;
+    u.adj = j.adj + (int64_t)un * j.adj_stride;
+    u.ck = j.ck ? j.ck + (int64_t)un * j.ck_stride : nullptr;
+    u.save = j.save + (int64_t)un * j.save_stride;
+    u.grad = j.gpriv + (int64_t)un * j.grad_stride;
+    u.vel = j.vel + (int64_t)m * j.fe;
+    u.wav = j.wav;
+    u.syn = j.syn + ((int64_t)m * j.ns + sh) * j.nt * j.nr;
+    u.obs = j.obs + (int64_t)sh * j.nt * j.nr;
+    units[j.unit0 + un] = u;
+}
+
+// ---------------------------------------------------------------------------
+// Reductions
+
+// out[m][p] += sum over shots of the per-unit gradients, shot order fixed.
+__global__ void grad_reduce_kernel(const float* __restrict__ gpriv, int64_t grad_stride, float* __restrict__ out,
+                                   int ns, int ni, int nsample)
+{
+    const int64_t total = (int64_t)nsample * ni;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const int m = (int)(i / ni);
+        const int p = (int)(i - (int64_t)m * ni);
+        double acc = (double)out[i];
+        for (int s = 0; s < ns; ++s) acc += (double)gpriv[((int64_t)m * ns + s) * grad_stride + p];
+        out[i] = (float)acc;
+    }
+}
+
+// Per-sample partial sums of (syn - obs)^2 in double; fixed order.
+constexpr int kMisfitThreads = 256;
+__global__ __launch_bounds__(kMisfitThreads) void misfit_partial_kernel(
+    const float* __restrict__ syn, const float* __restrict__ obs, int64_t len, int nblk,
+    double* __restrict__ partial)
+{
+    __shared__ double red[kMisfitThreads];
+    const int m = blockIdx.y, b = blockIdx.x;
+    const int64_t chunk = (len + nblk - 1) / nblk;
+    const int64_t lo = (int64_t)b * chunk;
+    const int64_t hi = min(len, lo + chunk);
+    const float* sm = syn + (int64_t)m * len;
+    double acc = 0.0;
+    for (int64_t i = lo + threadIdx.x; i < hi; i += kMisfitThreads) {
+        const double d = (double)(sm[i] - obs[i]);
+        acc += d * d;
+    }
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int w = kMisfitThreads / 2; w > 0; w >>= 1) {
+        if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[(int64_t)m * nblk + b] = red[0];
+}
+
+__global__ void misfit_final_kernel(const double* __restrict__ partial, int nblk, int nsample,
+                                    double* __restrict__ misfit)
+{
+    const int m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= nsample) return;
+    double acc = 0.0;
+    for (int b = 0; b < nblk; ++b) acc += partial[(int64_t)m * nblk + b];
+    misfit[m] = 0.5 * acc;
+}
+
+// ---------------------------------------------------------------------------
+// Host launchers
+
+#define GPVI_VARIANTS(X)  \
+    X(0, 64, 16, 256)     \
+    X(1, 64, 8, 256)      \
+    X(2, 128, 8, 256)     \
+    X(3, 64, 32, 256)
+
+void variant_tile(int variant, int* tx, int* tz)
+{
+    switch (variant) {
+#define X(V, TX, TZ, NTH) \
+    case V: *tx = TX; *tz = TZ; return;
+        GPVI_VARIANTS(X)
+#undef X
+    default: *tx = 0; *tz = 0; return;
+    }
+}
+
+int variant_threads(int variant)
+{
+    switch (variant) {
+#define X(V, TX, TZ, NTH) \
+    case V: return NTH;
+        GPVI_VARIANTS(X)
+#undef X
+    default: return 0;
+    }
+}
+
+template <int LC>
+static hipError_t launch_single_lc(int variant, int grid, hipStream_t st, const UnitDesc& u, int s)
+{
+    switch (variant) {
+#define X(V, TX, TZ, NTH)                                                                        \
+    case V:                                                                                      \
+        hipLaunchKernelGGL((step_single_kernel<LC, TX, TZ, NTH>), dim3(grid), dim3(NTH), 0, st, u, s); \
+        return hipGetLastError();
+        GPVI_VARIANTS(X)
+#undef X
+    default: return hipErrorInvalidValue;
+    }
+}
+
+hipError_t launch_step_single(int variant, int lc, int grid, hipStream_t st, const UnitDesc& u, int s)
+{
+    switch (lc) {
+    case 4: return launch_single_lc<4>(variant, grid, st, u, s);
+    case 5: return launch_single_lc<5>(variant, grid, st, u, s);
+    case 6: return launch_single_lc<6>(variant, grid, st, u, s);
+    case 7: return launch_single_lc<7>(variant, grid, st, u, s);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+template <int LC>
+static hipError_t launch_multi_lc(int variant, int grid, hipStream_t st, const UnitDesc* units,
+                                  const TileRef* tiles, const StepCtl* ctl, int step_off)
+{
+    switch (variant) {
+#define X(V, TX, TZ, NTH)                                                                  \
+    case V:                                                                                \
+        hipLaunchKernelGGL((step_multi_kernel<LC, TX, TZ, NTH>), dim3(grid), dim3(NTH), 0, st, \
+                           units, tiles, ctl, step_off);                                   \
+        return hipGetLastError();
+        GPVI_VARIANTS(X)
+#undef X
+    default: return hipErrorInvalidValue;
+    }
+}
+
+hipError_t launch_step_multi(int variant, int lc, int grid, hipStream_t st, const UnitDesc* units,
+                             const TileRef* tiles, const StepCtl* ctl, int step_off)
+{
+    switch (lc) {
+    case 4: return launch_multi_lc<4>(variant, grid, st, units, tiles, ctl, step_off);
+    case 5: return launch_multi_lc<5>(variant, grid, st, units, tiles, ctl, step_off);
+    case 6: return launch_multi_lc<6>(variant, grid, st, units, tiles, ctl, step_off);
+    case 7: return launch_multi_lc<7>(variant, grid, st, units, tiles, ctl, step_off);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+hipError_t launch_advance(hipStream_t st, StepCtl* ctl, int n)
+{
+    hipLaunchKernelGGL(advance_kernel, dim3(1), dim3(64), 0, st, ctl, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_ctl_set(hipStream_t st, StepCtl* ctl, int step, const int* ntiles4, const int* off4)
+{
+    hipLaunchKernelGGL(ctl_set_kernel, dim3(1), dim3(64), 0, st, ctl, step, ntiles4[0], ntiles4[1],
+                       ntiles4[2], ntiles4[3], off4[0], off4[1], off4[2], off4[3]);
+    return hipGetLastError();
+}
+
+hipError_t launch_pad_velocity(hipStream_t st, const float* vin, float* vpad, int nx, int nz, int pml,
+                               int ntx, int ntz, int pitch, int64_t fe, int nsample)
+{
+    const int64_t total = (int64_t)ntz * pitch * nsample;
+    const int grid = (int)std::min<int64_t>((total + 255) / 256, 65536);
+    hipLaunchKernelGGL(pad_velocity_kernel, dim3(grid), dim3(256), 0, st, vin, vpad, nx, nz, pml, ntx, ntz,
+                       pitch, fe, nsample);
+    return hipGetLastError();
+}
+
+hipError_t launch_ricker(hipStream_t st, float* wav, int nt, float dt, float f0)
+{
+    hipLaunchKernelGGL(ricker_kernel, dim3((nt + 255) / 256), dim3(256), 0, st, wav, nt, dt, f0);
+    return hipGetLastError();
+}
+
+hipError_t launch_build_units(hipStream_t st, const JobDesc& j, UnitDesc* units, TileRef* tiles)
+{
+    const int64_t ntile = (int64_t)j.nsample * j.ns * j.tiles_x * j.tiles_z;
+    const int64_t nunit = (int64_t)j.nsample * j.ns;
+    const int64_t work = std::max(ntile, nunit);
+    const int grid = (int)std::min<int64_t>((work + 255) / 256, 65536);
+    hipLaunchKernelGGL(build_units_kernel, dim3(std::max(grid, (int)((nunit + 255) / 256))), dim3(256), 0, st, j,
+                       units, tiles);
+    return hipGetLastError();
+}
+
+hipError_t launch_grad_reduce(hipStream_t st, const float* gpriv, int64_t grad_stride, float* out, int ns,
+                              int ni, int nsample)
+{
+    const int64_t total = (int64_t)nsample * ni;
+    const int grid = (int)std::min<int64_t>((total + 255) / 256, 65536);
+    hipLaunchKernelGGL(grad_reduce_kernel, dim3(grid), dim3(256), 0, st, gpriv, grad_stride, out, ns, ni, nsample);
+    return hipGetLastError();
+}
+
+int misfit_blocks(int64_t len)
+{
+    return (int)std::max<int64_t>(1, std::min<int64_t>(256, len / 4096));
+}
+
+hipError_t launch_misfit(hipStream_t st, const float* syn, const float* obs, int64_t len, int nsample,
+                         double* partial, double* misfit)
+{
+    const int nblk = misfit_blocks(len);
+    hipLaunchKernelGGL(misfit_partial_kernel, dim3(nblk, nsample), dim3(kMisfitThreads), 0, st, syn, obs, len,
+                       nblk, partial);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(misfit_final_kernel, dim3((nsample + 63) / 64), dim3(64), 0, st, partial, nblk, nsample,
+                       misfit);
+    return hipGetLastError();
+}
+
+}  // namespace gpvi

@@ -1,0 +1,80 @@
+"""Drop-in for the reference's Cython extension ``aco2d``
+(geopvi/forward/fwi2d/pyacofwi2D.pyx:11-48).
+
+Same functions, arguments, dtype rules and outputs; the work runs on the GPU
+through the plain solver of libgeopvi_fwi2d.so (one sample, shots in order,
+one launch per time step), so the gradient accumulates shot after shot in the
+reference's order.
+"""
+from __future__ import annotations
+
+import sys
+
+import numpy as np
+import torch
+
+from . import solver
+from .params import FWIParams
+
+
+def _check_buffer(a, dtype, name):
+    # The Cython signature types these as C-contiguous 1-D buffers and
+    # rejects anything else with ValueError ("Buffer dtype mismatch ...").
+    if not isinstance(a, np.ndarray):
+        raise TypeError(f"Argument '{name}' has incorrect type (expected numpy.ndarray)")
+    if a.dtype != dtype:
+        raise ValueError(f"Buffer dtype mismatch, expected '{np.dtype(dtype).name}' but got '{a.dtype.name}'")
+    if a.ndim != 1:
+        raise ValueError(f"Buffer has wrong number of dimensions (expected 1, got {a.ndim})")
+    if not a.flags.c_contiguous:
+        raise ValueError("ndarray is not C-contiguous")
+
+
+def _device() -> torch.device:
+    if not torch.cuda.is_available():
+        raise RuntimeError("geopvi_amd: no HIP device visible (the FWI-2D solver runs on the GPU only)")
+    return torch.device("cuda", torch.cuda.current_device())
+
+
+def fwi(vel, record_obs, paramfile="./input/input_param.txt", is_fwi=1, verbose=0):
+    """Synthetic records and velocity gradient of one model.
+
+    vel: float64 (nz*nx,); record_obs: float32 (ns*nt*nr,).
+    Returns (record_syn float32 (len(record_obs),), grad float32 (len(vel),))."""
+    _check_buffer(vel, np.float64, "vel")
+    _check_buffer(record_obs, np.float32, "record_obs")
+    if np.isnan(vel).any():
+        print("NaN occured in python")
+        sys.exit()
+    params = FWIParams.read(paramfile)
+    if vel.shape[0] != params.ni:
+        raise ValueError(f"vel has {vel.shape[0]} values, the parameter file describes {params.ni}")
+    if record_obs.shape[0] != params.record_len:
+        raise ValueError(f"record_obs has {record_obs.shape[0]} values, expected ns*nt*nr = {params.record_len}")
+    if verbose:
+        print(f"fwi2d: nx={params.nx} nz={params.nz} nt={params.nt} ns={params.ns} "
+              f"dx={params.dx} dz={params.dz} dt={params.dt}")
+    dev = _device()
+    vel_f = torch.from_numpy(np.float32(vel)).to(dev)
+    obs = torch.from_numpy(record_obs).to(dev)
+    syn, grad = solver.fwi_plain(params, vel_f, obs, run_fwi=int(is_fwi))
+    return syn.cpu().numpy(), grad.cpu().numpy()
+
+
+def forward(vel, dim=1, paramfile="./input/input_param.txt", verbose=0):
+    """Synthetic records of one model, float32 of length ``dim`` (ns*nt*nr
+    values are written; any tail stays zero)."""
+    _check_buffer(vel, np.float64, "vel")
+    if np.isnan(vel).any():
+        print("NaN occured in python")
+        sys.exit()
+    params = FWIParams.read(paramfile)
+    if vel.shape[0] != params.ni:
+        raise ValueError(f"vel has {vel.shape[0]} values, the parameter file describes {params.ni}")
+    if dim < params.record_len:
+        raise ValueError(f"dim={dim} is smaller than ns*nt*nr = {params.record_len}")
+    dev = _device()
+    syn = solver.forward_plain(params, torch.from_numpy(np.float32(vel)).to(dev))
+    out = np.zeros(dim, dtype=np.float32)
+    out[: params.record_len] = syn.cpu().numpy()
+    return out
