@@ -104,11 +104,76 @@ int validate(const gpvi_fwi2d_params* p)
     return 0;
 }
 
+// ---------------------------------------------------------------------------
+// Constant-divisor division (see cdiv_fast in fwi2d_step.h): for a divisor
+// b in [2^-16, 2^16] with y = RN(1/b), q = RN(q0 + RN(a - b*q0)*y), q0 = RN(a*y),
+// equals RN(a/b) for a == 0 or 2^-80 <= |a| < 2^80 (Markstein; no
+// intermediate underflow or overflow there).  Division by b is invariant
+// under a -> 2a in that range, so checking every significand of one binade
+// checks the whole range; the result is cached per divisor.
+
+__attribute__((target("fma"))) static bool markstein_binade_fma(float b, float y)
+{
+    for (uint32_t m = 0; m < (1u << 23); ++m) {
+        float a;
+        const uint32_t bits = 0x3f800000u | m;
+        std::memcpy(&a, &bits, 4);
+        volatile float vb = b;
+        const float q = a / vb;
+        const float q0 = a * y;
+        const float r = __builtin_fmaf(-q0, b, a);
+        const float q1 = __builtin_fmaf(r, y, q0);
+        if (q != q1) return false;
+    }
+    return true;
+}
+
+static bool markstein_binade_soft(float b, float y)
+{
+    for (uint32_t m = 0; m < (1u << 23); m += 97) {
+        float a;
+        const uint32_t bits = 0x3f800000u | m;
+        std::memcpy(&a, &bits, 4);
+        volatile float vb = b;
+        const float q = a / vb;
+        const float q0 = a * y;
+        const float r = std::fma(-q0, b, a);
+        const float q1 = std::fma(r, y, q0);
+        if (q != q1) return false;
+    }
+    return true;
+}
+
+std::mutex g_div_mu;
+std::map<uint32_t, bool> g_div_ok;
+
+bool fastdiv_ok(float b, float* recip)
+{
+    volatile float one = 1.f;
+    *recip = one / b;
+    const float ab = std::fabs(b);
+    if (!(ab >= 0x1p-16f && ab <= 0x1p16f)) return false;
+    uint32_t key;
+    std::memcpy(&key, &b, 4);
+    std::lock_guard<std::mutex> lk(g_div_mu);
+    auto it = g_div_ok.find(key);
+    if (it != g_div_ok.end()) return it->second;
+    const bool ok = __builtin_cpu_supports("fma") ? markstein_binade_fma(b, *recip)
+                                                   : markstein_binade_soft(b, *recip);
+    g_div_ok[key] = ok;
+    return ok;
+}
+
+// Field buffers: ntz rows of `pitch` floats (pitch = ntx rounded up to the
+// 64-column strip width) with kPadRows margin rows before and after; `fe` is
+// the buffer stride and `origin` the offset of row 0 inside a buffer.
 struct Geom {
-    int lc, pml, ntx, ntz, pitch, nx, nz, ni, nt, k, nr, ns, nsave;
-    int variant, tile_x, tile_z, tiles_x, tiles_z, tpu, threads;
-    int64_t fe;
+    int lc, pml, ntx, ntz, pitch, nx, nz, ni, nt, k, nr, ns, nsave, pml0;
+    int variant, tile_x, tile_z, tiles_x, tiles_z, tpu, zseg;
+    int64_t fe, origin;
     float fd[8];
+    float rdx, rdz, rdt;
+    int fastdiv;
 };
 
 void make_geom(const gpvi_fwi2d_params* p, int variant, Geom* g)
@@ -117,21 +182,32 @@ void make_geom(const gpvi_fwi2d_params* p, int variant, Geom* g)
     g->pml = p->pml0 + g->lc;
     g->ntx = p->nx + 2 * g->pml;
     g->ntz = p->nz + 2 * g->pml;
-    g->pitch = (g->ntx + 31) / 32 * 32;
+    g->pitch = (g->ntx + kStripWidth - 1) / kStripWidth * kStripWidth;
     g->nx = p->nx; g->nz = p->nz; g->ni = p->nx * p->nz;
     g->nt = p->nt; g->k = p->nt_interval; g->nr = p->nr; g->ns = p->ns;
     g->nsave = (p->nt + p->nt_interval - 1) / p->nt_interval;
-    g->fe = (int64_t)g->pitch * g->ntz;
+    g->fe = (int64_t)g->pitch * (g->ntz + 2 * kPadRows);
+    g->origin = (int64_t)g->pitch * kPadRows;
     g->variant = variant;
     variant_tile(variant, &g->tile_x, &g->tile_z);
-    g->threads = variant_threads(variant);
+    g->zseg = g->tile_z;
     g->tiles_x = (g->ntx + g->tile_x - 1) / g->tile_x;
     g->tiles_z = (g->ntz + g->tile_z - 1) / g->tile_z;
     g->tpu = g->tiles_x * g->tiles_z;
+    g->pml0 = p->pml0;
     fd_coeffs(g->lc, g->fd);
+    const bool okx = fastdiv_ok(p->dx, &g->rdx);
+    const bool okz = fastdiv_ok(p->dz, &g->rdz);
+    const bool okt = fastdiv_ok(p->dt, &g->rdt);
+    g->fastdiv = (okx && okz && okt) ? 1 : 0;
+    const char* off = std::getenv("GPVI_NO_FASTDIV");  // forces the IEEE division path (tests)
+    if (off && off[0] && off[0] != '0') g->fastdiv = 0;
 }
 
 constexpr size_t kAlign = 256;
+// The plain solver launches one unit per step: short strip segments (8 rows)
+// give it the most wavefronts.
+constexpr int kPlainVariant = 3;
 
 struct Bump {
     char* base;
@@ -170,8 +246,10 @@ Sched make_sched(const Geom& g, int ckpt_req)
     return s;
 }
 
+// Field-buffer pointers of the layouts below point at row 0 (origin) of
+// their first buffer.
 struct BatchLayout {
-    float *vel, *wav, *fwd, *adj, *ck, *save, *gpriv;
+    float *vel, *wav, *fwd, *adj, *ck, *save, *gpriv, *zero;
     double* partial;
     UnitDesc* units;
     TileRef* tiles;
@@ -190,11 +268,13 @@ void batch_layout(const Geom& g, const Sched& sc, int nsample, bool tables, char
     L->ck_stride = sc.nseg > 1 ? rnd(2 * (int64_t)sc.nseg * g.fe) : 0;
     L->save_stride = rnd((int64_t)sc.nslot * g.ni);
     L->grad_stride = rnd(g.ni);
-    L->vel = b.take<float>((size_t)nsample * g.fe);
-    L->wav = b.take<float>(g.nt);
-    L->fwd = b.take<float>((size_t)(U * L->fwd_stride));
-    L->adj = b.take<float>((size_t)(U * L->adj_stride));
-    L->ck = sc.nseg > 1 ? b.take<float>((size_t)(U * L->ck_stride)) : nullptr;
+    auto org = [&](float* p) { return p ? p + g.origin : nullptr; };
+    L->vel = org(b.take<float>((size_t)nsample * g.fe));
+    L->zero = org(b.take<float>((size_t)g.fe));
+    L->wav = b.take<float>(g.nt + g.pml0 + 1);  // wavelet, then the ABC taper table
+    L->fwd = org(b.take<float>((size_t)(U * L->fwd_stride)));
+    L->adj = org(b.take<float>((size_t)(U * L->adj_stride)));
+    L->ck = sc.nseg > 1 ? org(b.take<float>((size_t)(U * L->ck_stride))) : nullptr;
     L->save = b.take<float>((size_t)(U * L->save_stride));
     L->gpriv = b.take<float>((size_t)(U * L->grad_stride));
     L->partial = b.take<double>((size_t)nsample * misfit_blocks((int64_t)g.ns * g.nt * g.nr));
@@ -223,6 +303,7 @@ JobDesc make_job(const gpvi_fwi2d_params* p, const Geom& g, const Sched& sc, int
     JobDesc j;
     std::memset(&j, 0, sizeof j);
     j.nsample = nsample; j.ns = g.ns; j.tiles_x = g.tiles_x; j.tiles_z = g.tiles_z;
+    j.zseg = g.zseg;
     j.tile0 = 0; j.unit0 = 0; j.ntx = g.ntx; j.ntz = g.ntz;
     j.pitch = g.pitch; j.nx = g.nx; j.nz = g.nz; j.pml = g.pml;
     j.pml0 = p->pml0; j.lc = g.lc; j.nt = g.nt; j.k = g.k;
@@ -240,20 +321,36 @@ JobDesc make_job(const gpvi_fwi2d_params* p, const Geom& g, const Sched& sc, int
     j.run_fwi = run_fwi ? 1 : 0;
     j.store = store;
     j.dt = p->dt; j.dx = p->dx; j.dz = p->dz;
+    j.rdx = g.rdx; j.rdz = g.rdz; j.rdt = g.rdt; j.fastdiv = g.fastdiv;
     for (int c = 0; c < 8; ++c) j.fd[c] = g.fd[c];
     j.fe = g.fe;
+    j.wtab = L.wav + g.nt;
     j.fwd_stride = L.fwd_stride; j.adj_stride = L.adj_stride; j.ck_stride = L.ck_stride;
     j.save_stride = L.save_stride; j.grad_stride = L.grad_stride;
     j.fwd = L.fwd; j.adj = L.adj; j.ck = L.ck; j.save = L.save; j.gpriv = L.gpriv;
-    j.vel = L.vel; j.wav = L.wav; j.syn = syn; j.obs = obs;
+    j.vel = L.vel; j.wav = L.wav; j.syn = syn; j.obs = obs; j.zero = L.zero;
     return j;
 }
 
-int auto_grid(int ntiles)
+// Per-job device set-up: padded models, wavelet and taper, the zero field.
+int job_setup(hipStream_t st, const gpvi_fwi2d_params* p, const Geom& g, const float* vel_inner, float* vel,
+              float* wav, float* zero, int nsample)
+{
+    HIP_TRY(launch_pad_velocity(st, vel_inner, vel, g.nx, g.nz, g.pml, g.ntx, g.ntz, g.pitch, g.fe, nsample));
+    HIP_TRY(launch_tables(st, wav, g.nt, p->dt, p->f0, wav + g.nt, g.pml0));
+    HIP_TRY(launch_zero(st, zero - g.origin, g.fe));
+    return 0;
+}
+
+// Workgroups of the multi-unit step: one wavefront per work item up to
+// 8 waves per SIMD on 256 CUs (2048 workgroups of 4 waves), a multiple of 8.
+int auto_grid(int nitems)
 {
     Tuning t = tuning();
     if (t.grid > 0) return std::max(8, t.grid / 8 * 8);
-    int g = (ntiles + 7) / 8 * 8;
+    const int waves_per_wg = kMultiThreads / 64;
+    int g = (nitems + waves_per_wg - 1) / waves_per_wg;
+    g = (g + 7) / 8 * 8;
     return std::max(8, std::min(g, 2048));
 }
 
@@ -396,27 +493,32 @@ int gpvi_fwi2d_tile_shape(int variant, int* tile_x, int* tile_z)
 // ---------------------------------------------------------------------------
 // Plain solver
 
-static void plain_layout(const Geom& g, char* base, float** vel, float** wav, float** fwd, float** adj,
-                         float** save, size_t* bytes, bool store)
+struct PlainLayout {
+    float *vel, *zero, *wav, *fwd, *adj, *save;
+    size_t bytes;
+};
+
+static void plain_layout(const Geom& g, char* base, bool store, PlainLayout* L)
 {
     Bump b{base};
-    *vel = b.take<float>(g.fe);
-    *wav = b.take<float>(g.nt);
-    *fwd = b.take<float>(3 * g.fe);
-    *adj = b.take<float>(3 * g.fe);
-    *save = store ? b.take<float>((size_t)g.nsave * g.ni) : nullptr;
-    *bytes = b.off;
+    auto org = [&](float* p) { return p ? p + g.origin : nullptr; };
+    L->vel = org(b.take<float>(g.fe));
+    L->zero = org(b.take<float>(g.fe));
+    L->wav = b.take<float>(g.nt + g.pml0 + 1);  // wavelet, then the ABC taper table
+    L->fwd = org(b.take<float>(3 * g.fe));
+    L->adj = org(b.take<float>(3 * g.fe));
+    L->save = store ? b.take<float>((size_t)g.nsave * g.ni) : nullptr;
+    L->bytes = b.off;
 }
 
 size_t gpvi_fwi2d_plain_workspace_bytes(const gpvi_fwi2d_params* p)
 {
     if (validate(p)) return 0;
     Geom g;
-    make_geom(p, 0, &g);
-    float *a, *b, *c, *d, *e;
-    size_t bytes;
-    plain_layout(g, nullptr, &a, &b, &c, &d, &e, &bytes, true);
-    return bytes;
+    make_geom(p, kPlainVariant, &g);
+    PlainLayout L;
+    plain_layout(g, nullptr, true, &L);
+    return L.bytes;
 }
 
 static UnitDesc plain_unit(const gpvi_fwi2d_params* p, const Geom& g)
@@ -428,8 +530,9 @@ static UnitDesc plain_unit(const gpvi_fwi2d_params* p, const Geom& g)
     u.nt = g.nt; u.k = g.k; u.nr = g.nr; u.dr = (g.nr > 1) ? p->dr : 1;
     u.r_ix0 = g.pml + p->nr0; u.r_iz = g.pml + p->depthr; u.s_iz = g.pml + p->depths;
     u.nseg = 1; u.ckpt = g.nt; u.last_len = g.nt; u.nslot = g.nsave;
-    u.tiles_x = g.tiles_x; u.tiles_z = g.tiles_z;
+    u.tiles_x = g.tiles_x; u.tiles_z = g.tiles_z; u.zseg = g.zseg;
     u.dt = p->dt; u.dx = p->dx; u.dz = p->dz;
+    u.rdx = g.rdx; u.rdz = g.rdz; u.rdt = g.rdt; u.fastdiv = g.fastdiv;
     for (int c = 0; c < 8; ++c) u.fd[c] = g.fd[c];
     u.field_elems = g.fe;
     return u;
@@ -444,17 +547,17 @@ static int plain_run(const gpvi_fwi2d_params* p, const float* vel_inner, float* 
     if (!vel_inner || !record_syn || (adjoint && (!record_obs || !grad)))
         return fail(GPVI_ERR_INVALID, "NULL buffer");
     Geom g;
-    make_geom(p, 0, &g);
-    float *vel, *wav, *fwd, *adj, *save;
-    size_t need;
-    plain_layout(g, nullptr, &vel, &wav, &fwd, &adj, &save, &need, adjoint);
-    if ((rc = check_workspace(workspace, workspace_bytes, need))) return rc;
-    plain_layout(g, (char*)workspace, &vel, &wav, &fwd, &adj, &save, &need, adjoint);
+    make_geom(p, kPlainVariant, &g);
+    PlainLayout L;
+    plain_layout(g, nullptr, adjoint, &L);
+    if ((rc = check_workspace(workspace, workspace_bytes, L.bytes))) return rc;
+    plain_layout(g, (char*)workspace, adjoint, &L);
     hipStream_t st = (hipStream_t)stream;
-    HIP_TRY(launch_pad_velocity(st, vel_inner, vel, g.nx, g.nz, g.pml, g.ntx, g.ntz, g.pitch, g.fe, 1));
-    HIP_TRY(launch_ricker(st, wav, g.nt, p->dt, p->f0));
+    if ((rc = job_setup(st, p, g, vel_inner, L.vel, L.wav, L.zero, 1))) return rc;
     UnitDesc u = plain_unit(p, g);
-    u.fwd = fwd; u.adj = adj; u.save = save; u.grad = grad; u.vel = vel; u.wav = wav;
+    u.fwd = L.fwd; u.adj = L.adj; u.save = L.save; u.grad = grad; u.vel = L.vel; u.wav = L.wav;
+    u.wtab = L.wav + g.nt;
+    u.zero = L.zero;
     u.run_fwi = run_fwi ? 1 : 0;
     u.grad_init = 0;            // accumulate into the caller's gradient, shot after shot
     u.store = adjoint ? 1 : 0;   // wavefield store
@@ -518,8 +621,7 @@ int gpvi_fwi2d_batch(const gpvi_fwi2d_params* p, int nsample, const float* vel_i
     batch_layout(g, sc, nsample, true, (char*)workspace, &L);
     hipStream_t st = (hipStream_t)stream;
     if (phase != GPVI_PHASE_ADJOINT) {
-        HIP_TRY(launch_pad_velocity(st, vel_inner, L.vel, g.nx, g.nz, g.pml, g.ntx, g.ntz, g.pitch, g.fe, nsample));
-        HIP_TRY(launch_ricker(st, L.wav, g.nt, p->dt, p->f0));
+        if ((rc = job_setup(st, p, g, vel_inner, L.vel, L.wav, L.zero, nsample))) return rc;
     }
     const int store = (phase == GPVI_PHASE_FORWARD && (flags & GPVI_FLAG_NO_STORE)) ? 0 : 1;
     JobDesc j = make_job(p, g, sc, nsample, L, record_obs, record_syn, run_fwi, phase, store);
@@ -734,13 +836,10 @@ int gpvi_engine_submit(gpvi_engine* e, const gpvi_fwi2d_params* p, int nsample, 
         engine_free_slots(e, first, jb.nunit);
         return fail(GPVI_ERR_HIP, "hipEventCreate: %s", hipGetErrorString(err));
     }
-    err = launch_pad_velocity(e->stream, vel_inner, jb.L.vel, jb.g.nx, jb.g.nz, jb.g.pml, jb.g.ntx, jb.g.ntz,
-                              jb.g.pitch, jb.g.fe, nsample);
-    if (err == hipSuccess) err = launch_ricker(e->stream, jb.L.wav, jb.g.nt, p->dt, p->f0);
-    if (err != hipSuccess) {
+    if ((rc = job_setup(e->stream, p, jb.g, vel_inner, jb.L.vel, jb.L.wav, jb.L.zero, nsample))) {
         engine_free_slots(e, first, jb.nunit);
         (void)hipEventDestroy(jb.done);
-        return fail(GPVI_ERR_HIP, "job setup: %s", hipGetErrorString(err));
+        return rc;
     }
     jb.j = make_job(p, jb.g, jb.sc, nsample, jb.L, record_obs, record_syn, run_fwi, GPVI_PHASE_ALL, 1);
     jb.j.unit0 = first;

@@ -24,6 +24,8 @@
 namespace gpvi {
 
 constexpr int kMaxLc = 7;       // solver half-width (configured Lc 3..6, plus one)
+constexpr int kPadRows = 16;    // margin rows above and below every field buffer
+constexpr int kRowBlock = 4;    // rows per prefetch block of a strip segment
 
 // Per-unit modes of one local step.
 enum StepMode : int { kFwd = 0, kRec = 1, kAdj = 2, kIdle = 3 };
@@ -36,17 +38,27 @@ struct UnitDesc {
     int nt, k, nr, dr;
     int r_ix0, r_iz, s_ix, s_iz;
     int nseg, ckpt, last_len, nslot;   // segments of the adjoint schedule
-    int tiles_x, tiles_z, start, s_off;  // local step s = global - start + s_off
+    int tiles_x, tiles_z, start, s_off;  // tiles_x: 64-column strips, tiles_z: row segments;
+                                         // local step s = global - start + s_off
+    int zseg, pad2, pad3, pad4;          // rows per strip segment (one wavefront's work item)
     int total, run_fwi, grad_init, store;  // grad_init: first imaging step writes; store: keep wavefields
-    float dt, dx, dz, pad1;
+    float dt, dx, dz, rdt;                 // rdx/rdz/rdt = RN(1/dx) ...: exact constant division
+    float rdx, rdz;
+    int fastdiv, pad1;                     // 1: the host verified the reciprocal division for dx, dz, dt
     float fd[8];
-    int64_t field_elems;  // pitch * ntz
-    float* fwd;           // 3 * field_elems
-    float* adj;           // 3 * field_elems
-    float* ck;            // nseg * 2 * field_elems (segment 0 slot unused)
+    int64_t field_elems;  // stride between field buffers: (ntz + 2*kPadRows) * pitch
+    const float* wtab;    // ABC taper w[i] = 1 - i/pml0, i < pml0
+    // Field pointers point at row 0 of a buffer that has kPadRows rows of
+    // margin before and after, so every row a strip segment touches is
+    // addressable without bounds checks (margin values never reach a grid
+    // point of the solution).
+    float* fwd;           // 3 field buffers
+    float* adj;           // 3 field buffers
+    float* ck;            // nseg * 2 field buffers (segment 0 slot unused)
     float* save;          // nslot * nx * nz, compact [slot][iz][ix] inner grid
     float* grad;          // nx * nz
-    const float* vel;     // field_elems, padded by edge replication
+    const float* vel;     // one field buffer, padded by edge replication
+    const float* zero;    // one all-zero field buffer (the fields before step 0)
     const float* wav;     // nt
     float* syn;           // nt * nr
     const float* obs;     // nt * nr

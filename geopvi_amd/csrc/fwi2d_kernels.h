@@ -22,9 +22,13 @@ struct JobDesc {
     int s_ix0, ds, s_iz, nseg;
     int ckpt, last_len, nslot, start;
     int s_off, total, run_fwi, store;
-    float dt, dx, dz, pad1;
+    int zseg, pad2, pad3, pad4;
+    float dt, dx, dz, rdt;
+    float rdx, rdz;
+    int fastdiv, pad1;
     float fd[8];
     int64_t fe;
+    const float* wtab;
     int64_t fwd_stride, adj_stride, ck_stride, save_stride, grad_stride;
     float* fwd;
     float* adj;
@@ -35,20 +39,26 @@ struct JobDesc {
     const float* wav;
     float* syn;
     const float* obs;
+    const float* zero;
 };
 
+// Launch variants = rows per strip segment (the work item of one wavefront);
+// strips are always 64 columns wide.
 void variant_tile(int variant, int* tx, int* tz);
-int variant_threads(int variant);
-constexpr int kNumVariants = 4;
+constexpr int kNumVariants = 6;
+constexpr int kStripWidth = 64;
+constexpr int kMultiThreads = 256;   // workgroup of the multi-unit step (4 independent waves)
+constexpr int kSingleThreads = 64;   // workgroup of the single-unit step (1 wave)
 
 hipError_t launch_step_single(int variant, int lc, int grid, hipStream_t st, const UnitDesc& u, int s);
 hipError_t launch_step_multi(int variant, int lc, int grid, hipStream_t st, const UnitDesc* units,
                              const TileRef* tiles, const StepCtl* ctl, int step_off);
 hipError_t launch_advance(hipStream_t st, StepCtl* ctl, int n);
 hipError_t launch_ctl_set(hipStream_t st, StepCtl* ctl, int step, const int* ntiles4, const int* off4);
+hipError_t launch_zero(hipStream_t st, float* p, int64_t n);
 hipError_t launch_pad_velocity(hipStream_t st, const float* vin, float* vpad, int nx, int nz, int pml,
                                int ntx, int ntz, int pitch, int64_t fe, int nsample);
-hipError_t launch_ricker(hipStream_t st, float* wav, int nt, float dt, float f0);
+hipError_t launch_tables(hipStream_t st, float* wav, int nt, float dt, float f0, float* wtab, int pml0);
 hipError_t launch_build_units(hipStream_t st, const JobDesc& j, UnitDesc* units, TileRef* tiles);
 hipError_t launch_grad_reduce(hipStream_t st, const float* gpriv, int64_t grad_stride, float* out, int ns,
                               int ni, int nsample);

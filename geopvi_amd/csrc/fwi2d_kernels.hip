@@ -8,42 +8,52 @@ namespace gpvi {
 // ---------------------------------------------------------------------------
 // Step kernels
 
-// One unit, one tile per workgroup (the plain per-shot solver).
-template <int LC, int TX, int TZ, int NTH>
+// One unit (the plain per-shot solver): one wavefront per strip segment.
+template <int LC, int NTH>
 __global__ __launch_bounds__(NTH) void step_single_kernel(UnitDesc u, int s)
 {
-    __shared__ float s_p1[TileCfg<LC, TX, TZ, NTH>::kLds];
-    process_tile<LC, TX, TZ, NTH>(u, s, blockIdx.x, s_p1);
+    constexpr int NW = NTH / 64;
+    __shared__ float rows[NW][kStripWidth + 2 * (LC - 1)];
+    // readfirstlane: the compiler then knows the wave index is uniform, so
+    // everything derived from it lives in scalar registers.
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int item = blockIdx.x * NW + wave;
+    if (item < u.tiles_x * u.tiles_z) process_strip<LC>(u, s, item, rows[wave]);
 }
 
-// Many units per launch.  A fixed grid walks the tile list; tiles are
-// split into 8 contiguous chunks, one per XCD (workgroups are dispatched to
-// XCDs round-robin), so halo neighbours of a tile tend to be read through
-// the same L2.  Step and tile count come from device memory so that a
-// captured graph can be replayed without re-instantiation.
-template <int LC, int TX, int TZ, int NTH>
+// Many units per launch.  A fixed grid of independent wavefronts walks the
+// work-item list (strip segments of every unit); the list is split into 8
+// contiguous chunks, one per XCD (workgroups are dispatched to XCDs
+// round-robin), so the rows a segment shares with its neighbours tend to be
+// read through the same L2.  Step and item count come from device memory so
+// a captured graph segment can be replayed without re-instantiation.
+template <int LC, int NTH>
 __global__ __launch_bounds__(NTH) void step_multi_kernel(const UnitDesc* __restrict__ units,
                                                          const TileRef* __restrict__ tiles,
                                                          const StepCtl* __restrict__ ctl, int step_off)
 {
-    __shared__ float s_p1[TileCfg<LC, TX, TZ, NTH>::kLds];
+    constexpr int NW = NTH / 64;
+    __shared__ float rows[NW][kStripWidth + 2 * (LC - 1)];
+    // readfirstlane: the wave index, the work item and the unit descriptor
+    // address are then known to be uniform (scalar loads, scalar registers).
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int lcs = LC - 4;
     const int ntiles = ctl->ntiles_lc[lcs];
     const int toff = ctl->tile_off[lcs];
     const int gstep = ctl->step + step_off;
-    const int G = gridDim.x;
     const int b = blockIdx.x;
-    const int xcd = b & 7, lb = b >> 3, gper = G >> 3;
+    const int xcd = b & 7;
+    const int lw = (b >> 3) * NW + wave;          // wave index within its XCD
+    const int gper = (gridDim.x >> 3) * NW;       // waves per XCD
     const int chunk = (ntiles + 7) >> 3;
     const int t_begin = xcd * chunk;
     const int t_end = min(ntiles, t_begin + chunk);
-    for (int t = t_begin + lb; t < t_end; t += gper) {
+    for (int t = t_begin + lw; t < t_end; t += gper) {
         const TileRef tr = tiles[toff + t];
         const UnitDesc& u = units[tr.unit];
         const int s = gstep - u.start + u.s_off;
         if (s < u.s_off || s >= u.s_off + u.total) continue;
-        process_tile<LC, TX, TZ, NTH>(u, s, tr.tile, s_p1);
-        __syncthreads();
+        process_strip<LC>(u, s, tr.tile, rows[wave]);
     }
 }
 
@@ -87,16 +97,27 @@ __global__ void pad_velocity_kernel(const float* __restrict__ vin, float* __rest
     }
 }
 
-// Ricker wavelet (fdtd2D_modelling.c:1084-1093): the squared argument is
-// rounded to float, the wavelet is formed in double and rounded to float.
-__global__ void ricker_kernel(float* __restrict__ wav, int nt, float dt, float f0)
+__global__ void fill_zero_kernel(float* __restrict__ p, int64_t n)
+{
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = 0.f;
+}
+
+// Per-job tables.  Ricker wavelet (fdtd2D_modelling.c:1084-1093): the
+// squared argument is rounded to float, the wavelet is formed in double and
+// rounded to float.  ABC taper w[i] = 1 - i/pml0 in double, stored as float
+// (fdtd2D_modelling.c:227-230).
+__global__ void tables_kernel(float* __restrict__ wav, int nt, float dt, float f0, float* __restrict__ wtab,
+                              int pml0)
 {
     const int it = blockIdx.x * blockDim.x + threadIdx.x;
-    if (it >= nt) return;
-    const float arg = (float)it * dt - 1.f / f0;
-    const double x = 3.1415926 * (double)f0 * (double)arg;
-    const float tau = (float)(x * x);
-    wav[it] = (float)((1.0 - 2.0 * (double)tau) * exp(-(double)tau));
+    if (it < nt) {
+        const float arg = (float)it * dt - 1.f / f0;
+        const double x = 3.1415926 * (double)f0 * (double)arg;
+        const float tau = (float)(x * x);
+        wav[it] = (float)((1.0 - 2.0 * (double)tau) * exp(-(double)tau));
+    }
+    if (it < pml0) wtab[it] = (float)(1.0 - 1.0 * (double)it / (double)pml0);
 }
 
 // Fill the unit table and tile list of one job from its descriptor.
@@ -120,16 +141,20 @@ __global__ void build_units_kernel(JobDesc j, UnitDesc* __restrict__ units, Tile
     u.r_ix0 = j.r_ix0; u.r_iz = j.r_iz; u.s_ix = j.s_ix0 + sh * j.ds; u.s_iz = j.s_iz;
     u.nseg = j.nseg; u.ckpt = j.ckpt; u.last_len = j.last_len; u.nslot = j.nslot;
     u.tiles_x = j.tiles_x; u.tiles_z = j.tiles_z; u.start = j.start; u.s_off = j.s_off;
+    u.zseg = j.zseg; u.pad2 = 0; u.pad3 = 0; u.pad4 = 0;
     u.total = j.total; u.run_fwi = j.run_fwi; u.grad_init = 1; u.store = j.store;
-    u.dt = j.dt; u.dx = j.dx; u.dz = j.dz; u.pad1 = 0.f;
+    u.dt = j.dt; u.dx = j.dx; u.dz = j.dz; u.rdt = j.rdt;
+    u.rdx = j.rdx; u.rdz = j.rdz; u.fastdiv = j.fastdiv; u.pad1 = 0;
     for (int c = 0; c < 8; ++c) u.fd[c] = j.fd[c];
     u.field_elems = j.fe;
+    u.wtab = j.wtab;
     u.fwd = j.fwd + (int64_t)un * j.fwd_stride;
     u.adj = j.adj + (int64_t)un * j.adj_stride;
     u.ck = j.ck ? j.ck + (int64_t)un * j.ck_stride : nullptr;
     u.save = j.save + (int64_t)un * j.save_stride;
     u.grad = j.gpriv + (int64_t)un * j.grad_stride;
     u.vel = j.vel + (int64_t)m * j.fe;
+    u.zero = j.zero;
     u.wav = j.wav;
     u.syn = j.syn + ((int64_t)m * j.ns + sh) * j.nt * j.nr;
     u.obs = j.obs + (int64_t)sh * j.nt * j.nr;
@@ -193,83 +218,49 @@ __global__ void misfit_final_kernel(const double* __restrict__ partial, int nblk
 // ---------------------------------------------------------------------------
 // Host launchers
 
-#define GPVI_VARIANTS(X)  \
-    X(0, 64, 16, 256)     \
-    X(1, 64, 8, 256)      \
-    X(2, 128, 8, 256)     \
-    X(3, 64, 32, 256)
+// Rows per strip segment of each launch variant.
+static const int kZseg[kNumVariants] = {32, 16, 64, 8, 24, 48};
 
 void variant_tile(int variant, int* tx, int* tz)
 {
-    switch (variant) {
-#define X(V, TX, TZ, NTH) \
-    case V: *tx = TX; *tz = TZ; return;
-        GPVI_VARIANTS(X)
-#undef X
-    default: *tx = 0; *tz = 0; return;
-    }
+    const bool ok = variant >= 0 && variant < kNumVariants;
+    *tx = ok ? kStripWidth : 0;
+    *tz = ok ? kZseg[variant] : 0;
 }
 
-int variant_threads(int variant)
-{
-    switch (variant) {
-#define X(V, TX, TZ, NTH) \
-    case V: return NTH;
-        GPVI_VARIANTS(X)
-#undef X
-    default: return 0;
-    }
-}
-
-template <int LC>
-static hipError_t launch_single_lc(int variant, int grid, hipStream_t st, const UnitDesc& u, int s)
-{
-    switch (variant) {
-#define X(V, TX, TZ, NTH)                                                                        \
-    case V:                                                                                      \
-        hipLaunchKernelGGL((step_single_kernel<LC, TX, TZ, NTH>), dim3(grid), dim3(NTH), 0, st, u, s); \
-        return hipGetLastError();
-        GPVI_VARIANTS(X)
-#undef X
-    default: return hipErrorInvalidValue;
-    }
-}
-
+// `grid` = workgroups of kSingleThreads threads (one wavefront each).
 hipError_t launch_step_single(int variant, int lc, int grid, hipStream_t st, const UnitDesc& u, int s)
 {
+    (void)variant;
     switch (lc) {
-    case 4: return launch_single_lc<4>(variant, grid, st, u, s);
-    case 5: return launch_single_lc<5>(variant, grid, st, u, s);
-    case 6: return launch_single_lc<6>(variant, grid, st, u, s);
-    case 7: return launch_single_lc<7>(variant, grid, st, u, s);
+    case 4: hipLaunchKernelGGL((step_single_kernel<4, kSingleThreads>), dim3(grid), dim3(kSingleThreads), 0, st, u, s); break;
+    case 5: hipLaunchKernelGGL((step_single_kernel<5, kSingleThreads>), dim3(grid), dim3(kSingleThreads), 0, st, u, s); break;
+    case 6: hipLaunchKernelGGL((step_single_kernel<6, kSingleThreads>), dim3(grid), dim3(kSingleThreads), 0, st, u, s); break;
+    case 7: hipLaunchKernelGGL((step_single_kernel<7, kSingleThreads>), dim3(grid), dim3(kSingleThreads), 0, st, u, s); break;
     default: return hipErrorInvalidValue;
     }
+    return hipGetLastError();
 }
 
 template <int LC>
-static hipError_t launch_multi_lc(int variant, int grid, hipStream_t st, const UnitDesc* units,
-                                  const TileRef* tiles, const StepCtl* ctl, int step_off)
+static hipError_t launch_multi_lc(int grid, hipStream_t st, const UnitDesc* units, const TileRef* tiles,
+                                  const StepCtl* ctl, int step_off)
 {
-    switch (variant) {
-#define X(V, TX, TZ, NTH)                                                                  \
-    case V:                                                                                \
-        hipLaunchKernelGGL((step_multi_kernel<LC, TX, TZ, NTH>), dim3(grid), dim3(NTH), 0, st, \
-                           units, tiles, ctl, step_off);                                   \
-        return hipGetLastError();
-        GPVI_VARIANTS(X)
-#undef X
-    default: return hipErrorInvalidValue;
-    }
+    hipLaunchKernelGGL((step_multi_kernel<LC, kMultiThreads>), dim3(grid), dim3(kMultiThreads), 0, st, units,
+                       tiles, ctl, step_off);
+    return hipGetLastError();
 }
 
+// `grid` = workgroups of kMultiThreads threads, a multiple of 8.
 hipError_t launch_step_multi(int variant, int lc, int grid, hipStream_t st, const UnitDesc* units,
                              const TileRef* tiles, const StepCtl* ctl, int step_off)
 {
+    (void)variant;
     switch (lc) {
-    case 4: return launch_multi_lc<4>(variant, grid, st, units, tiles, ctl, step_off);
-    case 5: return launch_multi_lc<5>(variant, grid, st, units, tiles, ctl, step_off);
-    case 6: return launch_multi_lc<6>(variant, grid, st, units, tiles, ctl, step_off);
-    case 7: return launch_multi_lc<7>(variant, grid, st, units, tiles, ctl, step_off);
+    case 4: return launch_multi_lc<4>(grid, st, units, tiles, ctl, step_off);
+    case 5: return launch_multi_lc<5>(grid, st, units, tiles, ctl, step_off);
+    case 6: return launch_multi_lc<6>(grid, st, units, tiles, ctl, step_off);
+    case 7: return launch_multi_lc<7>(grid, st, units, tiles, ctl, step_off);
     default: return hipErrorInvalidValue;
     }
 }
@@ -287,6 +278,13 @@ hipError_t launch_ctl_set(hipStream_t st, StepCtl* ctl, int step, const int* nti
     return hipGetLastError();
 }
 
+hipError_t launch_zero(hipStream_t st, float* p, int64_t n)
+{
+    const int grid = (int)std::max<int64_t>(1, std::min<int64_t>((n + 255) / 256, 16384));
+    hipLaunchKernelGGL(fill_zero_kernel, dim3(grid), dim3(256), 0, st, p, n);
+    return hipGetLastError();
+}
+
 hipError_t launch_pad_velocity(hipStream_t st, const float* vin, float* vpad, int nx, int nz, int pml,
                                int ntx, int ntz, int pitch, int64_t fe, int nsample)
 {
@@ -297,9 +295,10 @@ hipError_t launch_pad_velocity(hipStream_t st, const float* vin, float* vpad, in
     return hipGetLastError();
 }
 
-hipError_t launch_ricker(hipStream_t st, float* wav, int nt, float dt, float f0)
+hipError_t launch_tables(hipStream_t st, float* wav, int nt, float dt, float f0, float* wtab, int pml0)
 {
-    hipLaunchKernelGGL(ricker_kernel, dim3((nt + 255) / 256), dim3(256), 0, st, wav, nt, dt, f0);
+    const int n = std::max(nt, pml0);
+    hipLaunchKernelGGL(tables_kernel, dim3((n + 255) / 256), dim3(256), 0, st, wav, nt, dt, f0, wtab, pml0);
     return hipGetLastError();
 }
 

@@ -1,33 +1,89 @@
-// The fused time step of one tile of one unit (see fwi2d_device.h for the
-// data layout).  Arithmetic follows the reference's expression shapes and
-// float/double typing (geopvi/forward/fwi2d/fdtd2D_modelling.c), and the
-// library is compiled with -ffp-contract=off, so that a point's update
-// rounds exactly as the reference's float build does.
+// The fused time step of the 2-D acoustic FDTD solver for gfx950 (see
+// fwi2d_device.h for the data layout).
+//
+// Work decomposition: every wavefront independently owns one "strip
+// segment" of one unit -- 64 consecutive columns (one per lane) times `zseg`
+// consecutive rows -- and marches down it in blocks of kRowBlock rows.  Each
+// lane keeps p1 of its own column around the current block in registers
+// (the z-neighbours of the stencil), the current row with its x-halo is
+// staged in a wave-private LDS row (the x-neighbours), and every load of the
+// next block (p1, x-halo, p0, velocity) is issued before the current block
+// is computed.  Field buffers carry margin rows and a 64-column pitch, so
+// these loads need no bounds checks.  No workgroup barrier is needed, so
+// waves never wait for each other.
+//
+// Arithmetic follows the reference's expression shapes and float/double
+// typing (geopvi/forward/fwi2d/fdtd2D_modelling.c); the library is compiled
+// with -ffp-contract=off, so a point's update rounds exactly as the
+// reference's float build does.
 #pragma once
 
 #include "fwi2d_device.h"
 
 namespace gpvi {
 
-// ABC taper weight w[i] = 1 - i/pml0, formed in double and stored as float
-// (fdtd2D_modelling.c:227-230).
-__device__ __forceinline__ float abc_weight(int i, int pml0)
+// Global-address-space views: loads/stores through these compile to
+// global_load/global_store with a 32-bit vector offset.
+template <class T>
+__device__ __forceinline__ const __attribute__((address_space(1))) T* G(const T* p)
 {
-    return (float)(1.0 - (double)i / (double)pml0);
+    return (const __attribute__((address_space(1))) T*)p;
+}
+template <class T>
+__device__ __forceinline__ __attribute__((address_space(1))) T* G(T* p)
+{
+    return (__attribute__((address_space(1))) T*)p;
 }
 
-// Order-2 transmitting extrapolation at one point, inward stride d1 in the
-// LDS copy of p1 and d0 in global p0 (fdtd2D_modelling.c:840-846).
-__device__ __forceinline__ float abc_extrapolate(const float* sp, int d1, const float* p0, int64_t ip,
-                                                 int64_t d0, float t11, float t12, float t13)
+// ---------------------------------------------------------------------------
+// Division by a per-job constant b with y = RN(1/b): q0 = a*y, r = a - b*q0
+// (exact through the FMA), q = RN(q0 + r*y) is the correctly rounded a/b
+// (Markstein's theorem) whenever no intermediate underflows or overflows.
+// That holds for 2^-16 <= |b| <= 2^16 and a == 0 or 2^-80 <= |a| < 2^80;
+// other lanes take the IEEE division.  The host checks the divisor range and
+// verifies the identity over a full binade before it sets UnitDesc::fastdiv.
+__device__ __forceinline__ bool cdiv_in_range(float a)
 {
-    float a0 = 0.f, a1 = 0.f, a2 = 0.f, a3 = 0.f, a4 = 0.f;
-    if (p0) {
-        a0 = p0[ip]; a1 = p0[ip + d0]; a2 = p0[ip + 2 * d0]; a3 = p0[ip + 3 * d0]; a4 = p0[ip + 4 * d0];
+    const float aa = __builtin_fabsf(a);
+    return (aa < 0x1p80f) && (aa >= 0x1p-80f || aa == 0.f);
+}
+
+__device__ __forceinline__ float cdiv_fast(float a, float b, float y)
+{
+    const float q0 = a * y;
+    const float r = __builtin_fmaf(-q0, b, a);
+    return __builtin_fmaf(r, y, q0);
+}
+
+// (a / b) / b, as the reference writes d2p/dx/dx and (.)/dt/dt.
+__device__ __forceinline__ float div2_const(float a, float b, float y, bool fast)
+{
+    if (fast) {
+        const float q = cdiv_fast(cdiv_fast(a, b, y), b, y);
+        if (__builtin_expect(cdiv_in_range(a), 1)) return q;
     }
-    float h = 2.f * t11 * sp[0];
-    h = h + 2.f * t12 * sp[d1];
-    h = h + 2.f * t13 * sp[2 * d1];
+    return (a / b) / b;
+}
+
+__device__ __forceinline__ float div1_const(float a, float b, float y, bool fast)
+{
+    if (fast) {
+        const float q = cdiv_fast(a, b, y);
+        if (__builtin_expect(cdiv_in_range(a), 1)) return q;
+    }
+    return a / b;
+}
+
+// Order-2 transmitting extrapolation (fdtd2D_modelling.c:840-846): q1..q3
+// are p1 at 0, 1, 2 cells inward, a0..a4 are p0 at 0..4 cells inward.
+__device__ __forceinline__ float abc_extrapolate(float q1, float q2, float q3, const float* p0, int ip, int d0,
+                                                 float t11, float t12, float t13)
+{
+    const auto g = G(p0);
+    const float a0 = g[ip], a1 = g[ip + d0], a2 = g[ip + 2 * d0], a3 = g[ip + 3 * d0], a4 = g[ip + 4 * d0];
+    float h = 2.f * t11 * q1;
+    h = h + 2.f * t12 * q2;
+    h = h + 2.f * t13 * q3;
     h = h - t11 * t11 * a0;
     h = h - 2.f * t11 * t12 * a1;
     h = h - (t12 * t12 + 2.f * t11 * t13) * a2;
@@ -36,39 +92,29 @@ __device__ __forceinline__ float abc_extrapolate(const float* sp, int d1, const 
     return h;
 }
 
-// w*H + (1.0 - w)*U: float product, double blend, float store.
+// w*H + (1.0 - w)*U: float product, double blend, float store (fdtd.c:840-847).
 __device__ __forceinline__ float abc_blend(float w, float h, float u)
 {
     return (float)((double)(w * h) + (1.0 - (double)w) * (double)u);
 }
 
-template <int LC, int TX, int TZ, int NTH>
-struct TileCfg {
-    static constexpr int H = LC - 1;              // stencil halo
-    static constexpr int W = TX + 2 * H;          // LDS row length
-    static constexpr int HT = TZ + 2 * H;         // LDS rows
-    static constexpr int kLds = W * HT;
-    static constexpr int ZSTEP = NTH / TX;
-    static_assert(NTH % TX == 0, "threads must cover whole tile rows");
-    static_assert(TZ % ZSTEP == 0, "tile height must be a multiple of rows per pass");
+// Per-step, per-unit quantities shared by every point a wave updates.
+struct StepCtx {
+    const float* p0;  // field two steps back (UnitDesc::zero before the pass starts)
+    const float* p1;  // field one step back
+    float* out;
+    float* save;      // this step's store/imaging slice (nx*nz), or nullptr
+    float* ck0;       // checkpoint slices written this step, or nullptr
+    float* ck1;
+    int it, mode;
+    bool first_image;
 };
 
-template <int LC, int TX, int TZ, int NTH>
-__device__ __forceinline__ void process_tile(const UnitDesc& u, int s, int tile, float* s_p1)
+__device__ __forceinline__ StepCtx make_ctx(const UnitDesc& u, const StepInfo& si)
 {
-    using C = TileCfg<LC, TX, TZ, NTH>;
-    constexpr int H = C::H, W = C::W;
-    const StepInfo si = decode_step(u, s);
-    if (si.mode == kIdle) return;
-
-    const int tz = tile / u.tiles_x;
-    const int tx = tile - tz * u.tiles_x;
-    const int x0 = tx * TX, z0 = tz * TZ;
-    const int ntx = u.ntx, ntz = u.ntz, pitch = u.pitch;
     const int64_t fe = u.field_elems;
-
-    const float* pre0 = nullptr;
-    const float* pre1 = nullptr;
+    const float* pre0 = u.zero;
+    const float* pre1 = u.zero;
     float* base = u.fwd;
     if (si.mode == kAdj) {
         base = u.adj;
@@ -77,125 +123,228 @@ __device__ __forceinline__ void process_tile(const UnitDesc& u, int s, int tile,
         pre1 = pre0 + fe;
     }
     const FieldPtrs f = rotate_ptrs(base, fe, pre0, pre1, si.q);
-
-    // Stage p1 (tile + halo) in LDS; outside the padded grid reads as 0.
-    if (f.p1) {
-        for (int i = threadIdx.x; i < C::kLds; i += NTH) {
-            const int r = i / W, c = i - r * W;
-            const int gz = z0 - H + r, gx = x0 - H + c;
-            float val = 0.f;
-            if (gz >= 0 && gz < ntz && gx >= 0 && gx < ntx) val = f.p1[(int64_t)gz * pitch + gx];
-            s_p1[i] = val;
-        }
-    } else {
-        for (int i = threadIdx.x; i < C::kLds; i += NTH) s_p1[i] = 0.f;
-    }
-    __syncthreads();
-
-    const int it = si.it;
-    const float dt = u.dt, dx = u.dx, dz = u.dz;
-    const int pml = u.pml, pml0 = u.pml0;
+    StepCtx c;
+    c.p0 = f.p0;
+    c.p1 = f.p1;
+    c.out = f.out;
+    c.it = si.it;
+    c.mode = si.mode;
     const bool fwd_like = si.mode != kAdj;
-    const bool do_store = fwd_like && u.store && (it % u.k == 0) && (si.mode == kRec || u.nseg == 1);
-    const bool do_image = !fwd_like && (it % u.k == 0);
-    const bool do_ckpt = (si.mode == kFwd) && (u.nseg > 1) && (it > 0) && (it % u.ckpt == 0);
-    const int slot = (u.nseg == 1) ? it / u.k : (it - si.seg * u.ckpt) / u.k;
-    const int ni = u.nx * u.nz;
-    const bool first_image = u.grad_init && (it == ((u.nt - 1) / u.k) * u.k);
+    const bool on_k = (si.it % u.k) == 0;
+    const bool store = fwd_like && u.store && on_k && (si.mode == kRec || u.nseg == 1);
+    const bool image = !fwd_like && on_k;
+    const int slot = (u.nseg == 1) ? si.it / u.k : (si.it - si.seg * u.ckpt) / u.k;
+    c.save = (store || image) ? u.save + (int64_t)slot * u.nx * u.nz : nullptr;
+    const bool ckpt = (si.mode == kFwd) && (u.nseg > 1) && (si.it > 0) && (si.it % u.ckpt == 0);
+    if (ckpt) {
+        const int j = si.it / u.ckpt;
+        c.ck0 = u.ck + (int64_t)(2 * j) * fe;
+        c.ck1 = c.ck0 + fe;
+    } else {
+        c.ck0 = c.ck1 = nullptr;
+    }
+    c.first_image = u.grad_init && (si.it == ((u.nt - 1) / u.k) * u.k);
+    return c;
+}
 
-    const int lx = threadIdx.x % TX;
-    const int lz0 = threadIdx.x / TX;
-#pragma unroll
-    for (int lz = lz0; lz < TZ; lz += C::ZSTEP) {
-        const int ix = x0 + lx, iz = z0 + lz;
-        if (ix >= ntx || iz >= ntz) continue;
-        const int64_t ip = (int64_t)iz * pitch + ix;
-        const float* sp = s_p1 + (lz + H) * W + (lx + H);
-        const float p1c = sp[0];
-        const float p0c = f.p0 ? f.p0[ip] : 0.f;
-        const float v = u.vel[ip];
+// Everything after the stencil and ABC for one point: source or residual
+// injection, checkpoint, wavefield store / record (forward-like steps) or
+// imaging condition (adjoint steps), and the output store.
+__device__ __forceinline__ void finish_point(const UnitDesc& u, const StepCtx& c, float val, float p0c, float p1c,
+                                             float v, int ix, int iz, int ip, bool rec_row, bool src_row,
+                                             bool inner)
+{
+    const float dt = u.dt;
+    int ir = -1;
+    if (rec_row) {
+        const int d = ix - u.r_ix0;
+        if (d >= 0 && d % u.dr == 0 && d / u.dr < u.nr) ir = d / u.dr;
+    }
+    const int gi = (iz - u.pml) * u.nx + (ix - u.pml);
+    if (c.mode != kAdj) {
+        // S3: source after the ABC, then checkpoint, store and record.
+        if (src_row && ix == u.s_ix) val = val + G(u.wav)[c.it] * dt * dt * v * v;
+        if (c.ck0) {
+            G(c.ck0)[ip] = p0c;
+            G(c.ck1)[ip] = p1c;
+        }
+        if (c.save && inner) {
+            float sv = val;
+            if (u.run_fwi) sv = div2_const((val - 2.f * p1c) + p0c, dt, u.rdt, u.fastdiv != 0);
+            G(c.save)[gi] = sv;
+        }
+        if (c.mode == kFwd && ir >= 0) G(u.syn)[c.it * u.nr + ir] = val;
+    } else {
+        // S5: residual injection, then the imaging condition.
+        if (ir >= 0) {
+            const int ri = c.it * u.nr + ir;
+            const float res = u.run_fwi ? G(u.syn)[ri] - G(u.obs)[ri] : G(u.obs)[ri];
+            val = val + res * dt * dt * v * v;
+        }
+        if (c.save && inner) {
+            const float sv = G(c.save)[gi];
+            const float gprev = c.first_image ? 0.f : G(u.grad)[gi];
+            float gnew;
+            if (u.run_fwi) {
+                const double vd = (double)v;
+                const float v3 = (float)(vd * vd * vd);
+                const double term = -2.0 / (double)v3 * (double)sv * (double)val * (double)u.k * (double)dt;
+                gnew = (float)((double)gprev + term);
+            } else {
+                gnew = gprev + sv * val * (float)u.k * dt;
+            }
+            G(u.grad)[gi] = gnew;
+        }
+    }
+    G(c.out)[ip] = val;
+}
 
-        // S1: interior update; points outside it start from p1.
-        float val;
-        if (iz >= LC && iz < ntz - LC && ix >= LC && ix < ntx - LC) {
-            float dxx = u.fd[0] * p1c;
+
+// One wave: one strip segment of one unit at local step s.  `row` is the
+// wave's private LDS row of 64 + 2*(LC-1) floats.
+template <int LC>
+__device__ __forceinline__ void process_strip(const UnitDesc& u, int s, int item, float* row)
+{
+    constexpr int H = LC - 1;
+    constexpr int R = kRowBlock;
+    constexpr int NWIN = 2 * H + R;  // p1 rows a block needs in each lane's column
+    const StepInfo si = decode_step(u, s);
+    if (si.mode == kIdle) return;
+    const StepCtx c = make_ctx(u, si);
+
+    const int ntx = u.ntx, ntz = u.ntz, pitch = u.pitch, pml = u.pml;
+    const int seg = item / u.tiles_x;
+    const int x0 = (item - seg * u.tiles_x) * 64;
+    const int zb = seg * u.zseg;
+    const int ze = min(zb + u.zseg, ntz);
+    const int lane = threadIdx.x & 63;
+    const int ix = x0 + lane;
+    const bool lane_ok = ix < ntx;
+
+    // Halo lanes: lanes 0..H-1 also carry columns x0-H.., lanes H..2H-1
+    // columns x0+64..; their LDS slots are lane and 64+lane.  Other lanes
+    // load their own column again (the value is not used).
+    const bool halo_lane = lane < 2 * H;
+    const int hcol = lane < H ? x0 - H + lane : (halo_lane ? x0 + 64 + (lane - H) : ix);
+    const int hslot = lane < H ? lane : 64 + lane;
+
+    const bool strip_inside = x0 >= pml && x0 + 64 <= ntx - pml;
+    const bool lane_interior_x = ix >= LC && ix < ntx - LC;
+    const bool lane_inner_x = ix >= pml && ix < ntx - pml;
+    const bool lane_band_x = (ix >= LC && ix < pml) || (ix >= ntx - pml && ix < ntx - LC);
+    const bool fast = u.fastdiv != 0;
+    const auto P1 = G(c.p1);
+    const auto P0 = G(c.p0);
+    const auto V = G(u.vel);
+
+    float win[NWIN];  // p1 at rows z0-H .. z0+R+H-1 of this lane's column
+    float hal[R], p0r[R], vr[R];
 #pragma unroll
-            for (int m = 1; m < LC; ++m) dxx = dxx + u.fd[m] * (sp[m] + sp[-m]);
-            float dzz = u.fd[0] * p1c;
+    for (int j = 0; j < NWIN; ++j) win[j] = P1[(zb - H + j) * pitch + ix];
 #pragma unroll
-            for (int m = 1; m < LC; ++m) dzz = dzz + u.fd[m] * (sp[m * W] + sp[-m * W]);
-            const float vdt = v * dt;
-            val = (2.f * p1c - p0c) + (vdt * vdt) * ((dxx / dx) / dx + (dzz / dz) / dz);
-        } else {
-            val = p1c;
+    for (int r = 0; r < R; ++r) {
+        hal[r] = P1[(zb + r) * pitch + hcol];
+        p0r[r] = P0[(zb + r) * pitch + ix];
+        vr[r] = V[(zb + r) * pitch + ix];
+    }
+
+    for (int z0 = zb; z0 < ze; z0 += R) {
+        // Issue every load of the next block before computing this one.
+        const int z1 = z0 + R;
+        float nwin[R], nhal[R], np0[R], nv[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            nwin[r] = P1[(z1 + H + r) * pitch + ix];
+            nhal[r] = P1[(z1 + r) * pitch + hcol];
+            np0[r] = P0[(z1 + r) * pitch + ix];
+            nv[r] = V[(z1 + r) * pitch + ix];
         }
 
-        // S2: top, bottom, left, right, in that order.
-        const bool top = iz >= LC && iz < pml;
-        const bool bot = iz >= ntz - pml && iz < ntz - LC;
-        const bool lef = ix >= LC && ix < pml;
-        const bool rig = ix >= ntx - pml && ix < ntx - LC;
-        if (top || bot || lef || rig) {
-            const float sc = (v * dt) / dx;
-            const float t11 = ((2.f - sc) * (1.f - sc)) / 2.f;
-            const float t12 = sc * (2.f - sc);
-            const float t13 = (sc * (sc - 1.f)) / 2.f;
-            if (top)
-                val = abc_blend(abc_weight(iz - LC, pml0),
-                                abc_extrapolate(sp, W, f.p0, ip, pitch, t11, t12, t13), val);
-            if (bot)
-                val = abc_blend(abc_weight(ntz - LC - iz - 1, pml0),
-                                abc_extrapolate(sp, -W, f.p0, ip, -(int64_t)pitch, t11, t12, t13), val);
-            if (lef)
-                val = abc_blend(abc_weight(ix - LC, pml0),
-                                abc_extrapolate(sp, 1, f.p0, ip, 1, t11, t12, t13), val);
-            if (rig)
-                val = abc_blend(abc_weight(ntx - LC - ix - 1, pml0),
-                                abc_extrapolate(sp, -1, f.p0, ip, -1, t11, t12, t13), val);
-        }
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const int iz = z0 + r;
+            if (iz >= ze) break;
+            const float p1c = win[H + r];
+            const float p0c = p0r[r];
+            const float v = vr[r];
+            // Stage the row (centre + x-halo) in the wave's LDS row.
+            row[lane + H] = p1c;
+            if (halo_lane) row[hslot] = hal[r];
+            __builtin_amdgcn_wave_barrier();
+            const float* sp = row + lane + H;
 
-        // Receiver index of this point, if any.
-        int ir = -1;
-        if (iz == u.r_iz) {
-            const int d = ix - u.r_ix0;
-            if (d >= 0 && d % u.dr == 0 && d / u.dr < u.nr) ir = d / u.dr;
-        }
+            const bool row_interior = iz >= LC && iz < ntz - LC;
+            const bool row_inner = iz >= pml && iz < ntz - pml;
+            const bool rec_row = iz == u.r_iz;
+            const bool src_row = iz == u.s_iz && c.mode != kAdj;
+            const int ip = iz * pitch + ix;
 
-        if (fwd_like) {
-            // S3: source after the ABC, then store and record.
-            if (iz == u.s_iz && ix == u.s_ix) val = val + u.wav[it] * dt * dt * v * v;
-            if (do_ckpt) {
-                const int j = it / u.ckpt;
-                u.ck[(int64_t)(2 * j) * fe + ip] = p0c;
-                u.ck[(int64_t)(2 * j + 1) * fe + ip] = p1c;
-            }
-            if (do_store && iz >= pml && iz < ntz - pml && ix >= pml && ix < ntx - pml) {
-                const int64_t gi = (int64_t)slot * ni + (int64_t)(iz - pml) * u.nx + (ix - pml);
-                u.save[gi] = u.run_fwi ? ((val - 2.f * p1c) + p0c) / dt / dt : val;
-            }
-            if (si.mode == kFwd && ir >= 0) u.syn[(int64_t)it * u.nr + ir] = val;
-        } else {
-            // S5: residual injection, then the imaging condition.
-            if (ir >= 0) {
-                const int64_t ri = (int64_t)it * u.nr + ir;
-                const float res = u.run_fwi ? u.syn[ri] - u.obs[ri] : u.obs[ri];
-                val = val + res * dt * dt * v * v;
-            }
-            if (do_image && iz >= pml && iz < ntz - pml && ix >= pml && ix < ntx - pml) {
-                const int gi = (iz - pml) * u.nx + (ix - pml);
-                const float sv = u.save[(int64_t)slot * ni + gi];
-                const float gprev = first_image ? 0.f : u.grad[gi];
-                if (u.run_fwi) {
-                    const double vd = (double)v;
-                    const float v3 = (float)(vd * vd * vd);
-                    const double term = -2.0 / (double)v3 * (double)sv * (double)val * (double)u.k * (double)dt;
-                    u.grad[gi] = (float)((double)gprev + term);
-                } else {
-                    u.grad[gi] = gprev + sv * val * (float)u.k * dt;
+            float val = p1c;
+            if (strip_inside && row_inner) {
+                // Inner grid, away from every absorbing band.
+                float dxx = u.fd[0] * p1c;
+#pragma unroll
+                for (int m = 1; m < LC; ++m) dxx = dxx + u.fd[m] * (sp[m] + sp[-m]);
+                float dzz = u.fd[0] * p1c;
+#pragma unroll
+                for (int m = 1; m < LC; ++m) dzz = dzz + u.fd[m] * (win[H + r + m] + win[H + r - m]);
+                const float vdt = v * u.dt;
+                const float lap = div2_const(dxx, u.dx, u.rdx, fast) + div2_const(dzz, u.dz, u.rdz, fast);
+                val = (2.f * p1c - p0c) + (vdt * vdt) * lap;
+                finish_point(u, c, val, p0c, p1c, v, ix, iz, ip, rec_row, src_row, true);
+            } else if (lane_ok) {
+                if (row_interior && lane_interior_x) {
+                    float dxx = u.fd[0] * p1c;
+#pragma unroll
+                    for (int m = 1; m < LC; ++m) dxx = dxx + u.fd[m] * (sp[m] + sp[-m]);
+                    float dzz = u.fd[0] * p1c;
+#pragma unroll
+                    for (int m = 1; m < LC; ++m) dzz = dzz + u.fd[m] * (win[H + r + m] + win[H + r - m]);
+                    const float vdt = v * u.dt;
+                    const float lap = div2_const(dxx, u.dx, u.rdx, fast) + div2_const(dzz, u.dz, u.rdz, fast);
+                    val = (2.f * p1c - p0c) + (vdt * vdt) * lap;
                 }
+                // S2: top, bottom, left, right, in that order.
+                const bool top = iz >= LC && iz < pml;
+                const bool bot = iz >= ntz - pml && iz < ntz - LC;
+                if (top || bot || lane_band_x) {
+                    const float sc = div1_const(v * u.dt, u.dx, u.rdx, fast);
+                    const float t11 = ((2.f - sc) * (1.f - sc)) / 2.f;
+                    const float t12 = sc * (2.f - sc);
+                    const float t13 = (sc * (sc - 1.f)) / 2.f;
+                    const auto wt = G(u.wtab);
+                    if (top)
+                        val = abc_blend(wt[iz - LC],
+                                        abc_extrapolate(p1c, win[H + r + 1], win[H + r + 2], c.p0, ip, pitch, t11,
+                                                        t12, t13),
+                                        val);
+                    if (bot)
+                        val = abc_blend(wt[ntz - LC - iz - 1],
+                                        abc_extrapolate(p1c, win[H + r - 1], win[H + r - 2], c.p0, ip, -pitch, t11,
+                                                        t12, t13),
+                                        val);
+                    if (ix >= LC && ix < pml)
+                        val = abc_blend(wt[ix - LC], abc_extrapolate(p1c, sp[1], sp[2], c.p0, ip, 1, t11, t12, t13),
+                                        val);
+                    if (ix >= ntx - pml && ix < ntx - LC)
+                        val = abc_blend(wt[ntx - LC - ix - 1],
+                                        abc_extrapolate(p1c, sp[-1], sp[-2], c.p0, ip, -1, t11, t12, t13), val);
+                }
+                finish_point(u, c, val, p0c, p1c, v, ix, iz, ip, rec_row, src_row, row_inner && lane_inner_x);
             }
+            __builtin_amdgcn_wave_barrier();
         }
-        f.out[ip] = val;
+
+        // Slide the window to the next block.
+#pragma unroll
+        for (int j = 0; j < 2 * H; ++j) win[j] = win[j + R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            win[2 * H + r] = nwin[r];
+            hal[r] = nhal[r];
+            p0r[r] = np0[r];
+            vr[r] = nv[r];
+        }
     }
 }
 
