@@ -868,7 +868,10 @@ int gpvi_engine_advance(gpvi_engine* e, int nsteps)
     GraphKey key{e->units, e->tiles, e->ctl, e->grid, e->variant, mask, e->seg_steps, e->device};
     hipGraphExec_t exec;
     if ((rc = get_graph(key, &exec))) return rc;
-    const int reps = (nsteps + e->seg_steps - 1) / e->seg_steps;
+    long long pending = 0;  // never run past the last active job's last step
+    for (const auto& jb : e->active) pending = std::max(pending, jb.end_step - e->gstep);
+    const long long todo = std::min<long long>(nsteps, pending);
+    const int reps = (int)((todo + e->seg_steps - 1) / e->seg_steps);
     for (int r = 0; r < reps; ++r) {
         // Keep at most three segments queued ahead of the device.
         while (e->inflight.size() >= 3) {

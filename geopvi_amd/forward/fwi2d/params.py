@@ -11,6 +11,7 @@ from __future__ import annotations
 import ctypes
 import dataclasses
 import math
+import struct
 
 # (INI key, comment line) in file order; the comment lines are the ones the
 # reference writes (posterior.py:29-66) so files stay interchangeable.
@@ -71,6 +72,14 @@ class FWIParams:
     dz: float = 10.0
     dt: float = 0.001
     f0: float = 10.0
+
+    def __post_init__(self):
+        # The solver reads these as float32 (fdtd2D_modelling.c:60-63, read
+        # with %f); keep the float32 value so every layer sees the same number.
+        for name, _, _ in _FLOAT_FIELDS:
+            setattr(self, name, struct.unpack("f", struct.pack("f", float(getattr(self, name))))[0])
+        for name, _, _ in _INT_FIELDS:
+            setattr(self, name, int(getattr(self, name)))
 
     # --- derived sizes (fdtd2D_modelling.c:70-90) ---------------------------
     @property

@@ -30,7 +30,9 @@ import os
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-_LIB_PATH = os.path.join(_HERE, "libfwi2d_oracle.so")
+# GPVI_ORACLE_LIB points the checker at another build of the same source
+# (used to run the pin tests against deliberately mutated oracles).
+_LIB_PATH = os.environ.get("GPVI_ORACLE_LIB") or os.path.join(_HERE, "libfwi2d_oracle.so")
 _lib = None
 
 

@@ -1,0 +1,117 @@
+"""The reference-facing interfaces on the GPU: aco2d.fwi / aco2d.forward,
+Posterior (host and device paths), the autograd Function, and the
+host-pointer fwi_2D / forward_2D C entry points, each against the oracle."""
+import configparser
+import ctypes
+
+import numpy as np
+import pytest
+import torch
+
+from helpers import GRAD_RTOL, MISFIT_RTOL, rel_err, rel_l2
+from geopvi_amd import _lib, synth
+from geopvi_amd.forward.fwi2d import aco2d, solver
+from geopvi_amd.forward.fwi2d.params import FWIParams
+from geopvi_amd.forward.fwi2d.posterior import Posterior
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def case(oracle_mod, tmp_path_factory):
+    p = synth.workload_params("tiny")
+    vt = synth.gradient_anomaly_model(p.nz, p.nx, seed=8)
+    vs = synth.smooth(vt, 2.0)
+    obs = oracle_mod.forward(p, vt)
+    syn_o, grad_o = oracle_mod.fwi(p, vs, obs)
+    path = str(tmp_path_factory.mktemp("fwi") / "input_params.txt")
+    p.write(path)
+    cfg = configparser.ConfigParser()
+    cfg["FWI"] = {key: (str(getattr(p, name)) if name not in ("dx", "dz", "dt", "f0") else repr(getattr(p, name)))
+                  for name, key in [("nx", "nx"), ("nz", "nz"), ("pml0", "pml"), ("Lc", "Lc"),
+                                    ("laplace_slover", "laplace_slover"), ("ns", "ns"), ("nt", "nt"), ("ds", "ds"),
+                                    ("ns0", "ns0"), ("depths", "depths"), ("depthr", "depthr"), ("nr", "nr"),
+                                    ("dr", "dr"), ("nr0", "nr0"), ("nt_interval", "nt_interval"), ("dx", "dx"),
+                                    ("dz", "dz"), ("dt", "dt"), ("f0", "f0")]}
+    return p, vt, vs, obs, syn_o, grad_o, path, cfg
+
+
+def test_aco2d_fwi_and_forward(case):
+    p, vt, vs, obs, syn_o, grad_o, path, _ = case
+    syn, grad = aco2d.fwi(vs.reshape(-1).astype(np.float64), obs, paramfile=path)
+    assert syn.dtype == np.float32 and grad.dtype == np.float32
+    assert np.array_equal(syn, syn_o)
+    assert rel_l2(grad, grad_o) <= GRAD_RTOL
+    rec = aco2d.forward(vt.reshape(-1).astype(np.float64), dim=p.record_len + 5, paramfile=path)
+    assert np.array_equal(rec[:p.record_len], obs) and not rec[p.record_len:].any()
+    with pytest.raises(ValueError):
+        aco2d.fwi(vs.reshape(-1).astype(np.float32), obs, paramfile=path)  # float64 required, as in Cython
+    bad = vs.reshape(-1).astype(np.float64).copy()
+    bad[3] = np.nan
+    with pytest.raises(SystemExit):
+        aco2d.fwi(bad, obs, paramfile=path)
+
+
+def test_host_pointer_entry_points(case):
+    p, vt, vs, obs, syn_o, grad_o, path, _ = case
+    lib = _lib.load()
+    fp = ctypes.POINTER(ctypes.c_float)
+    vel = np.ascontiguousarray(vs.reshape(-1), np.float32)
+    syn = np.zeros(p.record_len, np.float32)
+    grad = np.zeros(p.ni, np.float32)
+    o = np.ascontiguousarray(obs)
+    lib.fwi_2D(path.encode(), vel.ctypes.data_as(fp), syn.ctypes.data_as(fp), o.ctypes.data_as(fp),
+               grad.ctypes.data_as(fp), 1, 0)
+    assert lib.gpvi_compat_last_status() == 0
+    assert np.array_equal(syn, syn_o) and rel_l2(grad, grad_o) <= GRAD_RTOL
+    rec = np.zeros(p.record_len, np.float32)
+    vt32 = np.ascontiguousarray(vt.reshape(-1), np.float32)
+    lib.forward_2D(path.encode(), vt32.ctypes.data_as(fp), rec.ctypes.data_as(fp), 1, 0)
+    assert np.array_equal(rec, obs)
+
+
+def test_posterior_host_and_device_paths(case, oracle_mod):
+    p, vt, vs, obs, syn_o, grad_o, path, cfg = case
+    mask = np.ones(p.ni, bool)
+    mask[: 2 * p.nx] = False  # first two rows fixed at vel_fixed
+    vel_fixed = 1500.0
+    post = Posterior(obs, cfg, vel_fixed=vel_fixed, sigma=0.5, log_prior=lambda x: torch.zeros(x.shape[0],
+                     dtype=x.dtype, device=x.device), mask=mask, paramfile=path)
+    rng = np.random.default_rng(1)
+    xs = np.stack([vs.reshape(-1)[mask] + 20 * rng.standard_normal(mask.sum()) for _ in range(3)])
+    # oracle reference of misfit and gradient per sample
+    ref_loss, ref_grad = [], []
+    for x in xs:
+        v = np.full(p.ni, vel_fixed)
+        v[mask] = x
+        s, g = oracle_mod.fwi(p, v.astype(np.float32), obs)
+        ref_loss.append(oracle_mod.misfit_f32(s, obs))
+        ref_grad.append(-g[mask])
+    x_cpu = torch.tensor(xs, dtype=torch.float64, requires_grad=True)
+    logp = post.log_prob(x_cpu)
+    logp.sum().backward()
+    for i in range(3):
+        assert rel_err(-logp[i].item() * 0.25, ref_loss[i]) <= MISFIT_RTOL
+        assert rel_l2(-x_cpu.grad[i].numpy() * 0.25, ref_grad[i]) <= GRAD_RTOL
+    # device path: same numbers, gradient through the autograd Function
+    x_dev = torch.tensor(xs, dtype=torch.float64, device="cuda", requires_grad=True)
+    logp_d = post.log_prob(x_dev)
+    logp_d.sum().backward()
+    np.testing.assert_allclose(logp_d.detach().cpu().numpy(), logp.detach().numpy(), rtol=1e-12)
+    np.testing.assert_allclose(x_dev.grad.cpu().numpy(), x_cpu.grad.numpy(), rtol=1e-6, atol=0)
+    # solver(): the reference's one-sample wrapper
+    v = np.full(p.ni, vel_fixed)
+    v[mask] = xs[0]
+    loss, g = post.solver(v)
+    assert rel_err(loss, ref_loss[0]) <= MISFIT_RTOL and rel_l2(g, ref_grad[0]) <= GRAD_RTOL
+
+
+@pytest.mark.parametrize("ckpt", [0, 50])
+def test_autograd_function_matches_oracle(case, ckpt):
+    p, vt, vs, obs, syn_o, grad_o, path, _ = case
+    vel = torch.tensor(vs.reshape(1, -1), dtype=torch.float32, device="cuda", requires_grad=True)
+    mis = solver.fwi2d_misfit(vel, torch.from_numpy(obs).cuda(), p, ckpt_interval=ckpt)
+    (3.0 * mis.sum()).backward()
+    m_ref = 0.5 * np.sum((syn_o.astype(np.float64) - obs) ** 2)
+    assert rel_err(mis.item(), m_ref) <= MISFIT_RTOL
+    assert rel_l2(vel.grad.cpu().numpy()[0], -3.0 * grad_o) <= GRAD_RTOL
