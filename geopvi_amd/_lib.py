@@ -36,6 +36,8 @@ SIGNATURES = {
     "gpvi_fwi2d_set_tuning": (_i, [_i, _i, _i]),
     "gpvi_fwi2d_get_tuning": (_i, [ctypes.POINTER(_i)] * 3),
     "gpvi_fwi2d_tile_shape": (_i, [_i, ctypes.POINTER(_i), ctypes.POINTER(_i)]),
+    "gpvi_fwi2d_set_time_blocking": (_i, [_i]),
+    "gpvi_fwi2d_get_time_blocking": (_i, []),
     "gpvi_engine_create": (_i, [_vp, _i, _i, ctypes.POINTER(_vp)]),
     "gpvi_engine_destroy": (_i, [_vp]),
     "gpvi_engine_submit": (_i, [_vp, _P, _i, _vp, _vp, _vp, _vp, _vp, _i, _i, _vp, _sz,
@@ -101,6 +103,14 @@ def get_tuning():
     v, g, s = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
     load().gpvi_fwi2d_get_tuning(ctypes.byref(v), ctypes.byref(g), ctypes.byref(s))
     return v.value, g.value, s.value
+
+
+def set_time_blocking(steps_per_launch: int) -> None:
+    check(load().gpvi_fwi2d_set_time_blocking(int(steps_per_launch)))
+
+
+def get_time_blocking() -> int:
+    return int(load().gpvi_fwi2d_get_time_blocking())
 
 
 def tile_shape(variant: int):

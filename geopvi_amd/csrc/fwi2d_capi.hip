@@ -41,9 +41,10 @@ int fail(int code, const char* fmt, ...)
     } while (0)
 
 struct Tuning {
-    int variant = 0;
+    int variant = 4;       // 24-row strip segments: fastest measured on the 20-shot 300x100 batch
     int grid = 0;          // 0: automatic
     int graph_steps = 32;  // steps per graph segment
+    int tb = 1;            // time steps per multi-unit launch (1 or 2)
 };
 Tuning g_tuning;
 std::mutex g_tuning_mu;
@@ -169,14 +170,16 @@ bool fastdiv_ok(float b, float* recip)
 // the buffer stride and `origin` the offset of row 0 inside a buffer.
 struct Geom {
     int lc, pml, ntx, ntz, pitch, nx, nz, ni, nt, k, nr, ns, nsave, pml0;
-    int variant, tile_x, tile_z, tiles_x, tiles_z, tpu, zseg;
+    int variant, tile_x, tile_z, tiles_x, tiles_z, tpu, zseg, tb, stripw;
     int64_t fe, origin;
     float fd[8];
     float rdx, rdz, rdt;
     int fastdiv;
 };
 
-void make_geom(const gpvi_fwi2d_params* p, int variant, Geom* g)
+// tb: time steps per launch.  With two, a strip's 64 lanes cover its
+// 64 - 2*(lc-1) owned columns plus the first step's halo on each side.
+void make_geom(const gpvi_fwi2d_params* p, int variant, int tb, Geom* g)
 {
     g->lc = p->Lc + 1;
     g->pml = p->pml0 + g->lc;
@@ -191,6 +194,9 @@ void make_geom(const gpvi_fwi2d_params* p, int variant, Geom* g)
     g->variant = variant;
     variant_tile(variant, &g->tile_x, &g->tile_z);
     g->zseg = g->tile_z;
+    g->tb = (tb == 2) ? 2 : 1;
+    g->stripw = (g->tb == 2) ? kStripWidth - 2 * (g->lc - 1) : kStripWidth;
+    g->tile_x = g->stripw;
     g->tiles_x = (g->ntx + g->tile_x - 1) / g->tile_x;
     g->tiles_z = (g->ntz + g->tile_z - 1) / g->tile_z;
     g->tpu = g->tiles_x * g->tiles_z;
@@ -263,8 +269,8 @@ void batch_layout(const Geom& g, const Sched& sc, int nsample, bool tables, char
     Bump b{base};
     const int64_t U = (int64_t)nsample * g.ns;
     auto rnd = [](int64_t n) { return (n + 63) / 64 * 64; };  // keep per-unit blocks 256-B aligned
-    L->fwd_stride = rnd(3 * g.fe);
-    L->adj_stride = rnd(3 * g.fe);
+    L->fwd_stride = rnd(4 * g.fe);
+    L->adj_stride = rnd(4 * g.fe);
     L->ck_stride = sc.nseg > 1 ? rnd(2 * (int64_t)sc.nseg * g.fe) : 0;
     L->save_stride = rnd((int64_t)sc.nslot * g.ni);
     L->grad_stride = rnd(g.ni);
@@ -310,14 +316,18 @@ JobDesc make_job(const gpvi_fwi2d_params* p, const Geom& g, const Sched& sc, int
     j.nr = g.nr; j.dr = (g.nr > 1) ? p->dr : 1; j.r_ix0 = g.pml + p->nr0; j.r_iz = g.pml + p->depthr;
     j.s_ix0 = g.pml + p->ns0; j.ds = p->ds; j.s_iz = g.pml + p->depths; j.nseg = sc.nseg;
     j.ckpt = sc.ckpt; j.last_len = sc.last_len; j.nslot = sc.nslot; j.start = 0;
-    const int back = (sc.nseg == 1) ? g.nt : 2 * g.nt;
+    // Steps are counted in macro steps (launches) of g.tb time steps.
+    const int fwd = macro_steps(g.nt, sc.nseg, sc.ckpt, sc.last_len, g.tb, true, false);
+    const int back = macro_steps(g.nt, sc.nseg, sc.ckpt, sc.last_len, g.tb, false, true);
     if (phase == GPVI_PHASE_FORWARD) {
-        j.s_off = 0; j.total = g.nt;
+        j.s_off = 0; j.total = fwd;
     } else if (phase == GPVI_PHASE_ADJOINT) {
-        j.s_off = g.nt; j.total = back;
+        j.s_off = fwd; j.total = back;
     } else {
-        j.s_off = 0; j.total = g.nt + back;
+        j.s_off = 0; j.total = fwd + back;
     }
+    j.tb = g.tb;
+    j.stripw = g.stripw;
     j.run_fwi = run_fwi ? 1 : 0;
     j.store = store;
     j.dt = p->dt; j.dx = p->dx; j.dz = p->dz;
@@ -474,6 +484,17 @@ int gpvi_fwi2d_set_tuning(int variant, int grid, int graph_steps)
     return 0;
 }
 
+int gpvi_fwi2d_set_time_blocking(int steps_per_launch)
+{
+    if (steps_per_launch != 1 && steps_per_launch != 2)
+        return fail(GPVI_ERR_INVALID, "steps_per_launch=%d: 1 or 2", steps_per_launch);
+    std::lock_guard<std::mutex> lk(g_tuning_mu);
+    g_tuning.tb = steps_per_launch;
+    return 0;
+}
+
+int gpvi_fwi2d_get_time_blocking(void) { return tuning().tb; }
+
 int gpvi_fwi2d_get_tuning(int* variant, int* grid, int* graph_steps)
 {
     Tuning t = tuning();
@@ -505,8 +526,8 @@ static void plain_layout(const Geom& g, char* base, bool store, PlainLayout* L)
     L->vel = org(b.take<float>(g.fe));
     L->zero = org(b.take<float>(g.fe));
     L->wav = b.take<float>(g.nt + g.pml0 + 1);  // wavelet, then the ABC taper table
-    L->fwd = org(b.take<float>(3 * g.fe));
-    L->adj = org(b.take<float>(3 * g.fe));
+    L->fwd = org(b.take<float>(4 * g.fe));
+    L->adj = org(b.take<float>(4 * g.fe));
     L->save = store ? b.take<float>((size_t)g.nsave * g.ni) : nullptr;
     L->bytes = b.off;
 }
@@ -515,7 +536,7 @@ size_t gpvi_fwi2d_plain_workspace_bytes(const gpvi_fwi2d_params* p)
 {
     if (validate(p)) return 0;
     Geom g;
-    make_geom(p, kPlainVariant, &g);
+    make_geom(p, kPlainVariant, 1, &g);
     PlainLayout L;
     plain_layout(g, nullptr, true, &L);
     return L.bytes;
@@ -547,7 +568,7 @@ static int plain_run(const gpvi_fwi2d_params* p, const float* vel_inner, float* 
     if (!vel_inner || !record_syn || (adjoint && (!record_obs || !grad)))
         return fail(GPVI_ERR_INVALID, "NULL buffer");
     Geom g;
-    make_geom(p, kPlainVariant, &g);
+    make_geom(p, kPlainVariant, 1, &g);
     PlainLayout L;
     plain_layout(g, nullptr, adjoint, &L);
     if ((rc = check_workspace(workspace, workspace_bytes, L.bytes))) return rc;
@@ -561,7 +582,9 @@ static int plain_run(const gpvi_fwi2d_params* p, const float* vel_inner, float* 
     u.run_fwi = run_fwi ? 1 : 0;
     u.grad_init = 0;            // accumulate into the caller's gradient, shot after shot
     u.store = adjoint ? 1 : 0;   // wavefield store
-    u.total = adjoint ? 2 * g.nt : g.nt;
+    u.tb = 1;
+    u.stripw = g.stripw;
+    u.total = macro_steps(g.nt, 1, g.nt, g.nt, 1, true, adjoint);
     const int64_t rec = (int64_t)g.nt * g.nr;
     for (int is = 0; is < g.ns; ++is) {
         u.s_ix = is * p->ds + g.pml + p->ns0;
@@ -593,7 +616,7 @@ size_t gpvi_fwi2d_batch_workspace_bytes(const gpvi_fwi2d_params* p, int nsample,
 {
     if (validate(p) || nsample < 1) return 0;
     Geom g;
-    make_geom(p, tuning().variant, &g);
+    make_geom(p, tuning().variant, tuning().tb, &g);
     Sched sc = make_sched(g, ckpt_interval);
     BatchLayout L;
     batch_layout(g, sc, nsample, true, nullptr, &L);
@@ -613,7 +636,7 @@ int gpvi_fwi2d_batch(const gpvi_fwi2d_params* p, int nsample, const float* vel_i
         return fail(GPVI_ERR_INVALID, "NULL buffer");
     const Tuning tn = tuning();
     Geom g;
-    make_geom(p, tn.variant, &g);
+    make_geom(p, tn.variant, tn.tb, &g);
     Sched sc = make_sched(g, ckpt_interval);
     BatchLayout L;
     batch_layout(g, sc, nsample, true, nullptr, &L);
@@ -677,7 +700,7 @@ struct EngineJob {
 struct gpvi_engine {
     hipStream_t stream;
     int device;
-    int max_units, max_tiles, grid, variant, seg_steps;
+    int max_units, max_tiles, grid, variant, seg_steps, tb;
     UnitDesc* units;
     TileRef* tiles;
     StepCtl* ctl;
@@ -758,6 +781,7 @@ int gpvi_engine_create(void* stream, int max_units, int max_tiles, gpvi_engine**
     e->max_units = max_units;
     e->max_tiles = max_tiles;
     e->variant = t.variant;
+    e->tb = t.tb;
     e->seg_steps = t.graph_steps;
     e->grid = t.grid > 0 ? std::max(8, t.grid / 8 * 8) : 2048;
     e->gstep = 0;
@@ -819,7 +843,7 @@ int gpvi_engine_submit(gpvi_engine* e, const gpvi_fwi2d_params* p, int nsample, 
     if (nsample < 1 || !vel_inner || !record_obs || !record_syn || !grad)
         return fail(GPVI_ERR_INVALID, "bad job arguments");
     EngineJob jb;
-    make_geom(p, e->variant, &jb.g);
+    make_geom(p, e->variant, e->tb, &jb.g);
     jb.sc = make_sched(jb.g, ckpt_interval);
     batch_layout(jb.g, jb.sc, nsample, false, nullptr, &jb.L);
     if ((rc = check_workspace(workspace, workspace_bytes, jb.L.bytes))) return rc;

@@ -1,11 +1,12 @@
 // Device-side data layout and the fused time-step of the 2-D acoustic
 // FDTD misfit/gradient solver (gfx950).
 //
-// One "unit" is one (velocity sample, shot) pair: its own three rotating
-// forward fields, three adjoint fields, checkpoints, wavefield store and
+// One "unit" is one (velocity sample, shot) pair: its own four rotating
+// forward fields, four adjoint fields, checkpoints, wavefield store and
 // private gradient, all in HBM.  Every launch advances a set of units by one
-// local time step each; every thread of a tile owns output points and does,
-// for each point, the whole per-step work of the reference solver:
+// macro step (one or two time steps) each; every lane owns a grid column and
+// does, for each of its points, the whole per-step work of the reference
+// solver:
 //
 //   interior Laplacian update   fdtd2D_modelling.c:756-780
 //   4 ordered ABC sweeps        fdtd2D_modelling.c:833-900
@@ -14,8 +15,8 @@
 //   imaging condition           fdtd2D_modelling.c:959-975
 //
 // The copy rotation of the reference (p0<-p1<-p2) becomes a rotation of
-// three buffer indices, with the reference's invariant p2 == p1 at the start
-// of a step turned into "a point the stencil does not update starts from p1".
+// buffer indices, with the reference's invariant p2 == p1 at the start of a
+// step turned into "a point the stencil does not update starts from p1".
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -31,17 +32,19 @@ constexpr int kRowBlock = 4;    // rows per prefetch block of a strip segment
 enum StepMode : int { kFwd = 0, kRec = 1, kAdj = 2, kIdle = 3 };
 
 // Device unit descriptor.  Field buffers are "pitched": ntz rows of `pitch`
-// floats (pitch >= ntx, a multiple of 32 floats so rows start 128-B aligned).
+// floats (pitch >= ntx, a multiple of 64 floats).
 struct UnitDesc {
     int ntx, ntz, pitch, nx;
     int nz, pml, pml0, lc;
     int nt, k, nr, dr;
     int r_ix0, r_iz, s_ix, s_iz;
     int nseg, ckpt, last_len, nslot;   // segments of the adjoint schedule
-    int tiles_x, tiles_z, start, s_off;  // tiles_x: 64-column strips, tiles_z: row segments;
-                                         // local step s = global - start + s_off
-    int zseg, pad2, pad3, pad4;          // rows per strip segment (one wavefront's work item)
-    int total, run_fwi, grad_init, store;  // grad_init: first imaging step writes; store: keep wavefields
+    int tiles_x, tiles_z, start, s_off;  // tiles_x: column strips, tiles_z: row segments;
+                                         // local macro step s = global - start + s_off
+    int zseg, tb, stripw, pad4;          // rows per strip segment (one wavefront's work item),
+                                         // time steps per macro (1 or 2), columns per strip
+    int total, run_fwi, grad_init, store;  // total macro steps; grad_init: first imaging step writes;
+                                           // store: keep wavefields
     float dt, dx, dz, rdt;                 // rdx/rdz/rdt = RN(1/dx) ...: exact constant division
     float rdx, rdz;
     int fastdiv, pad1;                     // 1: the host verified the reciprocal division for dx, dz, dt
@@ -52,8 +55,8 @@ struct UnitDesc {
     // margin before and after, so every row a strip segment touches is
     // addressable without bounds checks (margin values never reach a grid
     // point of the solution).
-    float* fwd;           // 3 field buffers
-    float* adj;           // 3 field buffers
+    float* fwd;           // 4 field buffers
+    float* adj;           // 4 field buffers
     float* ck;            // nseg * 2 field buffers (segment 0 slot unused)
     float* save;          // nslot * nx * nz, compact [slot][iz][ix] inner grid
     float* grad;          // nx * nz
@@ -81,60 +84,86 @@ struct TileRef {
 };
 
 // ---------------------------------------------------------------------------
-// Schedule: local step s -> (mode, it, q, segment).
-//   s in [0, nt)          : forward, it = s
-//   then, segments j = nseg-1 .. 0, each:
+// Schedule.  A unit advances by one "macro step" per launch: tb (1 or 2)
+// consecutive time steps of one pass.  Passes, in order:
+//   forward   it = 0 .. nt-1
+//   then segments j = nseg-1 .. 0, each:
 //     (nseg > 1) recompute it = jC .. jC+Lj-1 from checkpoint j
 //     adjoint   it = jC+Lj-1 .. jC
-// q is the pass-local step index used for the three-buffer rotation.
+// A pass of odd length ends with a single-step macro.  q is the pass-local
+// step index of the first step (used for the buffer rotation); `count` is
+// the number of steps (1 or tb) of the macro.
 struct StepInfo {
-    int mode, it, q, seg;
+    int mode, it, q, seg, count;
 };
+
+__host__ __device__ inline int cdiv_i(int a, int b) { return (a + b - 1) / b; }
+__host__ __device__ inline int min_i(int a, int b) { return a < b ? a : b; }
+
+// Number of macro steps of a unit (the launches it takes).
+__host__ __device__ inline int macro_steps(int nt, int nseg, int ckpt, int last_len, int tb, bool forward,
+                                           bool backward)
+{
+    int n = forward ? cdiv_i(nt, tb) : 0;
+    if (backward)
+        n += (nseg == 1) ? cdiv_i(nt, tb) : (nseg - 1) * 2 * cdiv_i(ckpt, tb) + 2 * cdiv_i(last_len, tb);
+    return n;
+}
 
 __host__ __device__ inline StepInfo decode_step(const UnitDesc& u, int s)
 {
-    StepInfo r{kIdle, 0, 0, 0};
+    StepInfo r{kIdle, 0, 0, 0, 0};
+    const int tb = u.tb;
     if (s < 0) return r;
-    if (s < u.nt) {
-        r.mode = kFwd; r.it = s; r.q = s; r.seg = (u.nseg > 1) ? s / u.ckpt : 0;
+    const int nf = cdiv_i(u.nt, tb);
+    if (s < nf) {
+        const int it = tb * s;
+        r.mode = kFwd; r.it = it; r.q = it; r.seg = (u.nseg > 1) ? it / u.ckpt : 0;
+        r.count = min_i(tb, u.nt - it);
         return r;
     }
-    int b = s - u.nt;
+    const int b = s - nf;
     if (u.nseg == 1) {
-        if (b >= u.nt) return r;
-        r.mode = kAdj; r.it = u.nt - 1 - b; r.q = b; r.seg = 0;
+        if (b >= nf) return r;
+        const int it = u.nt - 1 - tb * b;
+        r.mode = kAdj; r.it = it; r.q = u.nt - 1 - it; r.seg = 0; r.count = min_i(tb, u.nt - tb * b);
         return r;
     }
+    const int bl = 2 * cdiv_i(u.last_len, tb), bc = 2 * cdiv_i(u.ckpt, tb);
     int j, w, lj;
-    if (b < 2 * u.last_len) {
+    if (b < bl) {
         j = u.nseg - 1; w = b; lj = u.last_len;
     } else {
-        int b2 = b - 2 * u.last_len;
-        j = u.nseg - 2 - b2 / (2 * u.ckpt);
+        const int b2 = b - bl;
+        j = u.nseg - 2 - b2 / bc;
         if (j < 0) return r;
-        w = b2 % (2 * u.ckpt); lj = u.ckpt;
+        w = b2 % bc; lj = u.ckpt;
     }
+    const int h = cdiv_i(lj, tb);
     r.seg = j;
-    if (w < lj) {
-        r.mode = kRec; r.it = j * u.ckpt + w; r.q = w;
+    if (w < h) {
+        r.mode = kRec; r.it = j * u.ckpt + tb * w; r.q = tb * w; r.count = min_i(tb, lj - tb * w);
     } else {
-        r.mode = kAdj; r.it = j * u.ckpt + (lj - 1) - (w - lj); r.q = u.nt - 1 - r.it;
+        const int w2 = w - h;
+        r.mode = kAdj; r.it = j * u.ckpt + lj - 1 - tb * w2; r.q = u.nt - 1 - r.it;
+        r.count = min_i(tb, lj - tb * w2);
     }
     return r;
 }
 
-// Three-buffer rotation with a two-field pre-history (zero, or a checkpoint):
+// Four-buffer rotation with a two-field pre-history (zero, or a checkpoint):
 //   q = 0: p0 = pre0, p1 = pre1, out = B0
 //   q = 1: p0 = pre1, p1 = B0,   out = B1
-//   q >= 2: p0 = B[(q-2)%3], p1 = B[(q-1)%3], out = B[q%3]
+//   q >= 2: p0 = B[(q-2)%4], p1 = B[(q-1)%4], out = B[q%4]
+// With four buffers the second step of a macro (output B[(q+1)%4]) never
+// writes a buffer the macro's first step reads.
 struct FieldPtrs {
-    const float* p0;  // nullptr == all zero
+    const float* p0;
     const float* p1;
     float* out;
 };
 
-__host__ __device__ inline FieldPtrs rotate_ptrs(float* base, int64_t fe, const float* pre0,
-                                                 const float* pre1, int q)
+__host__ __device__ inline FieldPtrs rotate_ptrs(float* base, int64_t fe, const float* pre0, const float* pre1, int q)
 {
     FieldPtrs f;
     if (q == 0) {
@@ -142,9 +171,9 @@ __host__ __device__ inline FieldPtrs rotate_ptrs(float* base, int64_t fe, const 
     } else if (q == 1) {
         f.p0 = pre1; f.p1 = base; f.out = base + fe;
     } else {
-        f.p0 = base + fe * ((q - 2) % 3);
-        f.p1 = base + fe * ((q - 1) % 3);
-        f.out = base + fe * (q % 3);
+        f.p0 = base + fe * ((q - 2) & 3);
+        f.p1 = base + fe * ((q - 1) & 3);
+        f.out = base + fe * (q & 3);
     }
     return f;
 }

@@ -22,7 +22,7 @@ struct JobDesc {
     int s_ix0, ds, s_iz, nseg;
     int ckpt, last_len, nslot, start;
     int s_off, total, run_fwi, store;
-    int zseg, pad2, pad3, pad4;
+    int zseg, tb, stripw, pad4;
     float dt, dx, dz, rdt;
     float rdx, rdz;
     int fastdiv, pad1;

@@ -13,12 +13,12 @@ template <int LC, int NTH>
 __global__ __launch_bounds__(NTH) void step_single_kernel(UnitDesc u, int s)
 {
     constexpr int NW = NTH / 64;
-    __shared__ float rows[NW][kStripWidth + 2 * (LC - 1)];
+    __shared__ float rows[NW][2 * (kStripWidth + 2 * (LC - 1))];
     // readfirstlane: the compiler then knows the wave index is uniform, so
     // everything derived from it lives in scalar registers.
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int item = blockIdx.x * NW + wave;
-    if (item < u.tiles_x * u.tiles_z) process_strip<LC>(u, s, item, rows[wave]);
+    if (item < u.tiles_x * u.tiles_z) process_item<LC>(u, s, item, rows[wave]);
 }
 
 // Many units per launch.  A fixed grid of independent wavefronts walks the
@@ -33,7 +33,7 @@ __global__ __launch_bounds__(NTH) void step_multi_kernel(const UnitDesc* __restr
                                                          const StepCtl* __restrict__ ctl, int step_off)
 {
     constexpr int NW = NTH / 64;
-    __shared__ float rows[NW][kStripWidth + 2 * (LC - 1)];
+    __shared__ float rows[NW][2 * (kStripWidth + 2 * (LC - 1))];
     // readfirstlane: the wave index, the work item and the unit descriptor
     // address are then known to be uniform (scalar loads, scalar registers).
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -53,7 +53,7 @@ __global__ __launch_bounds__(NTH) void step_multi_kernel(const UnitDesc* __restr
         const UnitDesc& u = units[tr.unit];
         const int s = gstep - u.start + u.s_off;
         if (s < u.s_off || s >= u.s_off + u.total) continue;
-        process_strip<LC>(u, s, tr.tile, rows[wave]);
+        process_item<LC>(u, s, tr.tile, rows[wave]);
     }
 }
 
@@ -141,7 +141,7 @@ __global__ void build_units_kernel(JobDesc j, UnitDesc* __restrict__ units, Tile
     u.r_ix0 = j.r_ix0; u.r_iz = j.r_iz; u.s_ix = j.s_ix0 + sh * j.ds; u.s_iz = j.s_iz;
     u.nseg = j.nseg; u.ckpt = j.ckpt; u.last_len = j.last_len; u.nslot = j.nslot;
     u.tiles_x = j.tiles_x; u.tiles_z = j.tiles_z; u.start = j.start; u.s_off = j.s_off;
-    u.zseg = j.zseg; u.pad2 = 0; u.pad3 = 0; u.pad4 = 0;
+    u.zseg = j.zseg; u.tb = j.tb; u.stripw = j.stripw; u.pad4 = 0;
     u.total = j.total; u.run_fwi = j.run_fwi; u.grad_init = 1; u.store = j.store;
     u.dt = j.dt; u.dx = j.dx; u.dz = j.dz; u.rdt = j.rdt;
     u.rdx = j.rdx; u.rdz = j.rdz; u.fastdiv = j.fastdiv; u.pad1 = 0;

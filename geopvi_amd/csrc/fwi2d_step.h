@@ -98,21 +98,202 @@ __device__ __forceinline__ float abc_blend(float w, float h, float u)
     return (float)((double)(w * h) + (1.0 - (double)w) * (double)u);
 }
 
-// Per-step, per-unit quantities shared by every point a wave updates.
+// ---------------------------------------------------------------------------
+// Raw buffer access: a 128-bit scalar resource (base address, byte range,
+// 32-bit data format) plus a per-lane byte offset (VGPR) and a uniform
+// per-row byte offset (SGPR).  One instruction per access, no per-lane
+// address arithmetic.  Bases sit at the first margin row of a field buffer
+// (or the start of an inner-grid array) so every offset is non-negative.
+typedef __amdgpu_buffer_rsrc_t Rsrc;
+constexpr int kRsrcFormat = 0x00020000;  // DATA_FORMAT = 32 bit (gfx9 family)
+
+__device__ __forceinline__ Rsrc make_rsrc(const void* base)
+{
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), (short)0, 0x7fffffff, kRsrcFormat);
+}
+// The b32 builtins move 32-bit words (unsigned int): bit-cast, never convert.
+__device__ __forceinline__ float bload(Rsrc r, unsigned voff, int soff)
+{
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0));
+}
+__device__ __forceinline__ void bstore(float v, Rsrc r, unsigned voff, int soff)
+{
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned int, v), r, voff, soff, 0);
+}
+
+// ---------------------------------------------------------------------------
+// One time step of a macro, as seen by the points a wave updates.
 struct StepCtx {
-    const float* p0;  // field two steps back (UnitDesc::zero before the pass starts)
-    const float* p1;  // field one step back
-    float* out;
-    float* save;      // this step's store/imaging slice (nx*nz), or nullptr
-    float* ck0;       // checkpoint slices written this step, or nullptr
-    float* ck1;
     int it, mode;
-    bool first_image;
+    bool first_image;  // first imaging step of the adjoint pass: the gradient starts from 0
+    bool has_save;     // wavefield store (forward-like) or imaging (adjoint) at this step
+    bool has_ck;       // checkpoint (p0, p1) written at this step
+    const float* p0g;  // p0 of this step in global memory (inward reads of the ABC)
+    Rsrc out, save, ck0, ck1;
 };
 
-__device__ __forceinline__ StepCtx make_ctx(const UnitDesc& u, const StepInfo& si)
+__device__ __forceinline__ StepCtx make_step(const UnitDesc& u, int mode, int it, int seg, const float* p0g,
+                                             float* out, int64_t margin)
 {
+    StepCtx c;
+    c.it = it;
+    c.mode = mode;
+    const bool fwd_like = mode != kAdj;
+    const bool on_k = (it % u.k) == 0;
+    const bool store = fwd_like && u.store && on_k && (mode == kRec || u.nseg == 1);
+    const bool image = !fwd_like && on_k;
+    c.has_save = store || image;
+    if (c.has_save) {
+        const int slot = (u.nseg == 1) ? it / u.k : (it - seg * u.ckpt) / u.k;
+        c.save = make_rsrc(u.save + (int64_t)slot * u.nx * u.nz);
+    }
+    c.has_ck = (mode == kFwd) && (u.nseg > 1) && (it > 0) && (it % u.ckpt == 0);
+    if (c.has_ck) {
+        const int j = it / u.ckpt;
+        c.ck0 = make_rsrc(u.ck + (int64_t)(2 * j) * u.field_elems - margin);
+        c.ck1 = make_rsrc(u.ck + (int64_t)(2 * j + 1) * u.field_elems - margin);
+    }
+    c.first_image = u.grad_init && (it == ((u.nt - 1) / u.k) * u.k);
+    c.p0g = p0g;
+    c.out = make_rsrc(out - margin);
+    return c;
+}
+
+// The value of one point after the stencil, the ABC and the source or
+// residual injection.  zn[H + m] is p1 at row iz + m (|m| <= H, this lane's
+// column), sp[m] p1 at column ix + m of row iz (the LDS row).  INSIDE: every
+// lane of the wave is an inner-grid point away from the absorbing bands.
+template <int LC, bool INSIDE>
+__device__ __forceinline__ float point_value(const UnitDesc& u, const StepCtx& c, int ix, int iz,
+                                             const float (&zn)[2 * LC - 1], const float* sp, float p0c, float v,
+                                             bool src_row, bool rec_row)
+{
+    constexpr int H = LC - 1;
+    const bool fast = u.fastdiv != 0;
+    const float p1c = zn[H];
+    const int ntx = u.ntx, ntz = u.ntz, pml = u.pml;
+    float val = p1c;
+    if (INSIDE || (iz >= LC && iz < ntz - LC && ix >= LC && ix < ntx - LC)) {
+        // S1: (2*p1 - p0) + (v*dt)^2 * (d2x/dx/dx + d2z/dz/dz)
+        float dxx = u.fd[0] * p1c;
+#pragma unroll
+        for (int m = 1; m < LC; ++m) dxx = dxx + u.fd[m] * (sp[m] + sp[-m]);
+        float dzz = u.fd[0] * p1c;
+#pragma unroll
+        for (int m = 1; m < LC; ++m) dzz = dzz + u.fd[m] * (zn[H + m] + zn[H - m]);
+        const float vdt = v * u.dt;
+        const float lap = div2_const(dxx, u.dx, u.rdx, fast) + div2_const(dzz, u.dz, u.rdz, fast);
+        val = (2.f * p1c - p0c) + (vdt * vdt) * lap;
+    }
+    if (!INSIDE) {
+        // S2: top, bottom, left, right, in that order.
+        const bool top = iz >= LC && iz < pml;
+        const bool bot = iz >= ntz - pml && iz < ntz - LC;
+        const bool lef = ix >= LC && ix < pml;
+        const bool rig = ix >= ntx - pml && ix < ntx - LC;
+        if (top || bot || lef || rig) {
+            const int ip = iz * u.pitch + ix;
+            const float sc = div1_const(v * u.dt, u.dx, u.rdx, fast);
+            const float t11 = ((2.f - sc) * (1.f - sc)) / 2.f;
+            const float t12 = sc * (2.f - sc);
+            const float t13 = (sc * (sc - 1.f)) / 2.f;
+            const auto wt = G(u.wtab);
+            if (top)
+                val = abc_blend(wt[iz - LC],
+                                abc_extrapolate(p1c, zn[H + 1], zn[H + 2], c.p0g, ip, u.pitch, t11, t12, t13), val);
+            if (bot)
+                val = abc_blend(wt[ntz - LC - iz - 1],
+                                abc_extrapolate(p1c, zn[H - 1], zn[H - 2], c.p0g, ip, -u.pitch, t11, t12, t13), val);
+            if (lef)
+                val = abc_blend(wt[ix - LC], abc_extrapolate(p1c, sp[1], sp[2], c.p0g, ip, 1, t11, t12, t13), val);
+            if (rig)
+                val = abc_blend(wt[ntx - LC - ix - 1], abc_extrapolate(p1c, sp[-1], sp[-2], c.p0g, ip, -1, t11, t12, t13),
+                                val);
+        }
+    }
+    const float dt = u.dt;
+    if (c.mode != kAdj) {
+        // S3: source after the ABC.
+        if (src_row && ix == u.s_ix) val = val + G(u.wav)[c.it] * dt * dt * v * v;
+    } else if (rec_row) {
+        // S5: residual injection at the receivers.
+        const int d = ix - u.r_ix0;
+        if (d >= 0 && d % u.dr == 0 && d / u.dr < u.nr) {
+            const int ri = c.it * u.nr + d / u.dr;
+            const float res = u.run_fwi ? G(u.syn)[ri] - G(u.obs)[ri] : G(u.obs)[ri];
+            val = val + res * dt * dt * v * v;
+        }
+    }
+    return val;
+}
+
+// Outputs of one owned point: checkpoint, wavefield store and record
+// (forward-like steps) or imaging condition (adjoint steps), and the field.
+// vo/ro: lane/row byte offsets in padded rows; go: byte offset of the row in
+// inner-grid arrays; inner: the point lies in the inner grid.
+__device__ __forceinline__ void store_point(const UnitDesc& u, const StepCtx& c, float val, float p0c, float p1c,
+                                            float v, int ix, unsigned vo, int ro, int go, Rsrc grad, bool rec_row,
+                                            bool inner)
+{
+    const float dt = u.dt;
+    const unsigned gvo = (unsigned)(ix - u.pml) * 4u;
+    if (c.mode != kAdj) {
+        if (c.has_ck) {
+            bstore(p0c, c.ck0, vo, ro);
+            bstore(p1c, c.ck1, vo, ro);
+        }
+        if (c.has_save && inner) {
+            float sv = val;
+            if (u.run_fwi) sv = div2_const((val - 2.f * p1c) + p0c, dt, u.rdt, u.fastdiv != 0);
+            bstore(sv, c.save, gvo, go);
+        }
+        if (c.mode == kFwd && rec_row) {
+            const int d = ix - u.r_ix0;
+            if (d >= 0 && d % u.dr == 0 && d / u.dr < u.nr) G(u.syn)[c.it * u.nr + d / u.dr] = val;
+        }
+    } else if (c.has_save && inner) {
+        const float sv = bload(c.save, gvo, go);
+        const float gprev = c.first_image ? 0.f : bload(grad, gvo, go);
+        float gnew;
+        if (u.run_fwi) {
+            const double vd = (double)v;
+            const float v3 = (float)(vd * vd * vd);
+            const double term = -2.0 / (double)v3 * (double)sv * (double)val * (double)u.k * (double)dt;
+            gnew = (float)((double)gprev + term);
+        } else {
+            gnew = gprev + sv * val * (float)u.k * dt;
+        }
+        bstore(gnew, grad, gvo, go);
+    }
+    bstore(val, c.out, vo, ro);
+}
+
+// One wavefront: one strip segment of one unit at local macro step s.
+//
+// tb == 1: lane l updates column x0 + l, rows [zb, ze), one step.
+// tb == 2: lane l covers column x0 - H + l.  Step A (the macro's first
+//   step) runs on every lane for rows [zb - H, ze + H); step B (the second)
+//   runs H rows behind it on the same lanes, reading A's results from a
+//   second register window (z-neighbours) and a second LDS row
+//   (x-neighbours).  Lanes H .. 63-H (the strip's 64 - 2H owned columns)
+//   store; A's values on the other lanes and rows only feed B, and are
+//   recomputed bit-identically by the neighbouring strips and segments.
+//
+// `lds` is the wave's private scratch: two rows of 64 + 2H floats.
+template <int LC>
+__device__ __forceinline__ void process_item(const UnitDesc& u, int s, int item, float* lds)
+{
+    constexpr int H = LC - 1;
+    constexpr int R = kRowBlock;
+    constexpr int NW = 2 * H + R;      // rows of a register window per block
+    constexpr int RW = 64 + 2 * H;     // LDS row length
+    const StepInfo si = decode_step(u, s);
+    if (si.mode == kIdle) return;
+    const bool pair = si.count == 2;
+
+    const int ntx = u.ntx, ntz = u.ntz, pitch = u.pitch, pml = u.pml;
     const int64_t fe = u.field_elems;
+    const int64_t margin = (int64_t)kPadRows * pitch;
     const float* pre0 = u.zero;
     const float* pre1 = u.zero;
     float* base = u.fwd;
@@ -123,227 +304,151 @@ __device__ __forceinline__ StepCtx make_ctx(const UnitDesc& u, const StepInfo& s
         pre1 = pre0 + fe;
     }
     const FieldPtrs f = rotate_ptrs(base, fe, pre0, pre1, si.q);
-    StepCtx c;
-    c.p0 = f.p0;
-    c.p1 = f.p1;
-    c.out = f.out;
-    c.it = si.it;
-    c.mode = si.mode;
-    const bool fwd_like = si.mode != kAdj;
-    const bool on_k = (si.it % u.k) == 0;
-    const bool store = fwd_like && u.store && on_k && (si.mode == kRec || u.nseg == 1);
-    const bool image = !fwd_like && on_k;
-    const int slot = (u.nseg == 1) ? si.it / u.k : (si.it - si.seg * u.ckpt) / u.k;
-    c.save = (store || image) ? u.save + (int64_t)slot * u.nx * u.nz : nullptr;
-    const bool ckpt = (si.mode == kFwd) && (u.nseg > 1) && (si.it > 0) && (si.it % u.ckpt == 0);
-    if (ckpt) {
-        const int j = si.it / u.ckpt;
-        c.ck0 = u.ck + (int64_t)(2 * j) * fe;
-        c.ck1 = c.ck0 + fe;
-    } else {
-        c.ck0 = c.ck1 = nullptr;
-    }
-    c.first_image = u.grad_init && (si.it == ((u.nt - 1) / u.k) * u.k);
-    return c;
-}
+    const StepCtx cA = make_step(u, si.mode, si.it, si.seg, f.p0, f.out, margin);
+    const int itB = si.mode == kAdj ? si.it - 1 : si.it + 1;
+    StepCtx cB;
+    if (pair) cB = make_step(u, si.mode, itB, si.seg, f.p1, base + fe * ((si.q + 1) & 3), margin);
 
-// Everything after the stencil and ABC for one point: source or residual
-// injection, checkpoint, wavefield store / record (forward-like steps) or
-// imaging condition (adjoint steps), and the output store.
-__device__ __forceinline__ void finish_point(const UnitDesc& u, const StepCtx& c, float val, float p0c, float p1c,
-                                             float v, int ix, int iz, int ip, bool rec_row, bool src_row,
-                                             bool inner)
-{
-    const float dt = u.dt;
-    int ir = -1;
-    if (rec_row) {
-        const int d = ix - u.r_ix0;
-        if (d >= 0 && d % u.dr == 0 && d / u.dr < u.nr) ir = d / u.dr;
-    }
-    const int gi = (iz - u.pml) * u.nx + (ix - u.pml);
-    if (c.mode != kAdj) {
-        // S3: source after the ABC, then checkpoint, store and record.
-        if (src_row && ix == u.s_ix) val = val + G(u.wav)[c.it] * dt * dt * v * v;
-        if (c.ck0) {
-            G(c.ck0)[ip] = p0c;
-            G(c.ck1)[ip] = p1c;
-        }
-        if (c.save && inner) {
-            float sv = val;
-            if (u.run_fwi) sv = div2_const((val - 2.f * p1c) + p0c, dt, u.rdt, u.fastdiv != 0);
-            G(c.save)[gi] = sv;
-        }
-        if (c.mode == kFwd && ir >= 0) G(u.syn)[c.it * u.nr + ir] = val;
-    } else {
-        // S5: residual injection, then the imaging condition.
-        if (ir >= 0) {
-            const int ri = c.it * u.nr + ir;
-            const float res = u.run_fwi ? G(u.syn)[ri] - G(u.obs)[ri] : G(u.obs)[ri];
-            val = val + res * dt * dt * v * v;
-        }
-        if (c.save && inner) {
-            const float sv = G(c.save)[gi];
-            const float gprev = c.first_image ? 0.f : G(u.grad)[gi];
-            float gnew;
-            if (u.run_fwi) {
-                const double vd = (double)v;
-                const float v3 = (float)(vd * vd * vd);
-                const double term = -2.0 / (double)v3 * (double)sv * (double)val * (double)u.k * (double)dt;
-                gnew = (float)((double)gprev + term);
-            } else {
-                gnew = gprev + sv * val * (float)u.k * dt;
-            }
-            G(u.grad)[gi] = gnew;
-        }
-    }
-    G(c.out)[ip] = val;
-}
-
-
-// One wave: one strip segment of one unit at local step s.  `row` is the
-// wave's private LDS row of 64 + 2*(LC-1) floats.
-template <int LC>
-__device__ __forceinline__ void process_strip(const UnitDesc& u, int s, int item, float* row)
-{
-    constexpr int H = LC - 1;
-    constexpr int R = kRowBlock;
-    constexpr int NWIN = 2 * H + R;  // p1 rows a block needs in each lane's column
-    const StepInfo si = decode_step(u, s);
-    if (si.mode == kIdle) return;
-    const StepCtx c = make_ctx(u, si);
-
-    const int ntx = u.ntx, ntz = u.ntz, pitch = u.pitch, pml = u.pml;
     const int seg = item / u.tiles_x;
-    const int x0 = (item - seg * u.tiles_x) * 64;
+    const int x0 = (item - seg * u.tiles_x) * u.stripw;
     const int zb = seg * u.zseg;
     const int ze = min(zb + u.zseg, ntz);
     const int lane = threadIdx.x & 63;
-    const int ix = x0 + lane;
-    const bool lane_ok = ix < ntx;
-
-    // Halo lanes: lanes 0..H-1 also carry columns x0-H.., lanes H..2H-1
-    // columns x0+64..; their LDS slots are lane and 64+lane.  Other lanes
-    // load their own column again (the value is not used).
+    const int loff = (u.tb == 2) ? H : 0;         // first owned lane
+    const int ix = x0 - loff + lane;              // this lane's column
+    const bool lane_ok = ix >= 0 && ix < ntx;
+    const bool owned = lane_ok && lane >= loff && lane < loff + u.stripw;
+    const int cx0 = x0 - loff - H;                // column of LDS slot 0
+    const unsigned li = lane + H;                 // this lane's LDS slot
+    const unsigned vo = li * 4u;
     const bool halo_lane = lane < 2 * H;
-    const int hcol = lane < H ? x0 - H + lane : (halo_lane ? x0 + 64 + (lane - H) : ix);
-    const int hslot = lane < H ? lane : 64 + lane;
+    const unsigned hslot = lane < H ? lane : 64 + lane;
+    const unsigned ho = hslot * 4u;
+    const int pitch4 = pitch * 4;
+    auto row_off = [&](int z) { return (z + kPadRows) * pitch4 + cx0 * 4; };
+    // Every lane of the wave inner and away from the bands (for both steps).
+    const bool strip_inside = (x0 - loff) >= pml && (x0 - loff + 64) <= ntx - pml;
+    const bool inner_x = ix >= pml && ix < ntx - pml;
 
-    const bool strip_inside = x0 >= pml && x0 + 64 <= ntx - pml;
-    const bool lane_interior_x = ix >= LC && ix < ntx - LC;
-    const bool lane_inner_x = ix >= pml && ix < ntx - pml;
-    const bool lane_band_x = (ix >= LC && ix < pml) || (ix >= ntx - pml && ix < ntx - LC);
-    const bool fast = u.fastdiv != 0;
-    const auto P1 = G(c.p1);
-    const auto P0 = G(c.p0);
-    const auto V = G(u.vel);
+    const Rsrc rp1 = make_rsrc(f.p1 - margin);
+    const Rsrc rp0 = make_rsrc(f.p0 - margin);
+    const Rsrc rv = make_rsrc(u.vel - margin);
+    const Rsrc rg = make_rsrc(u.grad);
+    float* row1 = lds;
+    float* row2 = lds + RW;
 
-    float win[NWIN];  // p1 at rows z0-H .. z0+R+H-1 of this lane's column
-    float hal[R], p0r[R], vr[R];
+    // Rows of step A: the segment, plus H on each side for a pair.
+    const int za_begin = pair ? zb - H : zb;
+    const int za_end = pair ? ze + H : ze;
+
+    float w1[NW];      // p1 at rows za0-H .. za0+R+H-1 (step A's z-neighbours)
+    float w2[NW];      // A's result at rows za0-2H .. za0+R-1 (step B's z-neighbours)
+    float vh[H + R];   // velocity at rows za0-H .. za0+R-1
+    float hal[R], p0r[R];
 #pragma unroll
-    for (int j = 0; j < NWIN; ++j) win[j] = P1[(zb - H + j) * pitch + ix];
+    for (int j = 0; j < NW; ++j) {
+        w1[j] = bload(rp1, vo, row_off(za_begin - H + j));
+        w2[j] = 0.f;
+    }
+#pragma unroll
+    for (int j = 0; j < H; ++j) vh[j] = 0.f;
 #pragma unroll
     for (int r = 0; r < R; ++r) {
-        hal[r] = P1[(zb + r) * pitch + hcol];
-        p0r[r] = P0[(zb + r) * pitch + ix];
-        vr[r] = V[(zb + r) * pitch + ix];
+        const int ro = row_off(za_begin + r);
+        hal[r] = halo_lane ? bload(rp1, ho, ro) : 0.f;
+        p0r[r] = bload(rp0, vo, ro);
+        vh[H + r] = bload(rv, vo, ro);
     }
 
-    for (int z0 = zb; z0 < ze; z0 += R) {
-        // Issue every load of the next block before computing this one.
-        const int z1 = z0 + R;
-        float nwin[R], nhal[R], np0[R], nv[R];
+    for (int za0 = za_begin; za0 < za_end; za0 += R) {
+        // Issue every load of the next block (if there is one) before
+        // computing this one.
+        const int za1 = za0 + R;
+        float n1[R], nh[R], n0[R], nv[R];
+        if (za1 < za_end) {
 #pragma unroll
-        for (int r = 0; r < R; ++r) {
-            nwin[r] = P1[(z1 + H + r) * pitch + ix];
-            nhal[r] = P1[(z1 + r) * pitch + hcol];
-            np0[r] = P0[(z1 + r) * pitch + ix];
-            nv[r] = V[(z1 + r) * pitch + ix];
-        }
-
-#pragma unroll
-        for (int r = 0; r < R; ++r) {
-            const int iz = z0 + r;
-            if (iz >= ze) break;
-            const float p1c = win[H + r];
-            const float p0c = p0r[r];
-            const float v = vr[r];
-            // Stage the row (centre + x-halo) in the wave's LDS row.
-            row[lane + H] = p1c;
-            if (halo_lane) row[hslot] = hal[r];
-            __builtin_amdgcn_wave_barrier();
-            const float* sp = row + lane + H;
-
-            const bool row_interior = iz >= LC && iz < ntz - LC;
-            const bool row_inner = iz >= pml && iz < ntz - pml;
-            const bool rec_row = iz == u.r_iz;
-            const bool src_row = iz == u.s_iz && c.mode != kAdj;
-            const int ip = iz * pitch + ix;
-
-            float val = p1c;
-            if (strip_inside && row_inner) {
-                // Inner grid, away from every absorbing band.
-                float dxx = u.fd[0] * p1c;
-#pragma unroll
-                for (int m = 1; m < LC; ++m) dxx = dxx + u.fd[m] * (sp[m] + sp[-m]);
-                float dzz = u.fd[0] * p1c;
-#pragma unroll
-                for (int m = 1; m < LC; ++m) dzz = dzz + u.fd[m] * (win[H + r + m] + win[H + r - m]);
-                const float vdt = v * u.dt;
-                const float lap = div2_const(dxx, u.dx, u.rdx, fast) + div2_const(dzz, u.dz, u.rdz, fast);
-                val = (2.f * p1c - p0c) + (vdt * vdt) * lap;
-                finish_point(u, c, val, p0c, p1c, v, ix, iz, ip, rec_row, src_row, true);
-            } else if (lane_ok) {
-                if (row_interior && lane_interior_x) {
-                    float dxx = u.fd[0] * p1c;
-#pragma unroll
-                    for (int m = 1; m < LC; ++m) dxx = dxx + u.fd[m] * (sp[m] + sp[-m]);
-                    float dzz = u.fd[0] * p1c;
-#pragma unroll
-                    for (int m = 1; m < LC; ++m) dzz = dzz + u.fd[m] * (win[H + r + m] + win[H + r - m]);
-                    const float vdt = v * u.dt;
-                    const float lap = div2_const(dxx, u.dx, u.rdx, fast) + div2_const(dzz, u.dz, u.rdz, fast);
-                    val = (2.f * p1c - p0c) + (vdt * vdt) * lap;
-                }
-                // S2: top, bottom, left, right, in that order.
-                const bool top = iz >= LC && iz < pml;
-                const bool bot = iz >= ntz - pml && iz < ntz - LC;
-                if (top || bot || lane_band_x) {
-                    const float sc = div1_const(v * u.dt, u.dx, u.rdx, fast);
-                    const float t11 = ((2.f - sc) * (1.f - sc)) / 2.f;
-                    const float t12 = sc * (2.f - sc);
-                    const float t13 = (sc * (sc - 1.f)) / 2.f;
-                    const auto wt = G(u.wtab);
-                    if (top)
-                        val = abc_blend(wt[iz - LC],
-                                        abc_extrapolate(p1c, win[H + r + 1], win[H + r + 2], c.p0, ip, pitch, t11,
-                                                        t12, t13),
-                                        val);
-                    if (bot)
-                        val = abc_blend(wt[ntz - LC - iz - 1],
-                                        abc_extrapolate(p1c, win[H + r - 1], win[H + r - 2], c.p0, ip, -pitch, t11,
-                                                        t12, t13),
-                                        val);
-                    if (ix >= LC && ix < pml)
-                        val = abc_blend(wt[ix - LC], abc_extrapolate(p1c, sp[1], sp[2], c.p0, ip, 1, t11, t12, t13),
-                                        val);
-                    if (ix >= ntx - pml && ix < ntx - LC)
-                        val = abc_blend(wt[ntx - LC - ix - 1],
-                                        abc_extrapolate(p1c, sp[-1], sp[-2], c.p0, ip, -1, t11, t12, t13), val);
-                }
-                finish_point(u, c, val, p0c, p1c, v, ix, iz, ip, rec_row, src_row, row_inner && lane_inner_x);
+            for (int r = 0; r < R; ++r) {
+                const int ro = row_off(za1 + r);
+                n1[r] = bload(rp1, vo, row_off(za1 + H + r));
+                nh[r] = halo_lane ? bload(rp1, ho, ro) : 0.f;
+                n0[r] = bload(rp0, vo, ro);
+                nv[r] = bload(rv, vo, ro);
             }
-            __builtin_amdgcn_wave_barrier();
+        } else {
+#pragma unroll
+            for (int r = 0; r < R; ++r) n1[r] = nh[r] = n0[r] = nv[r] = 0.f;
         }
 
-        // Slide the window to the next block.
-#pragma unroll
-        for (int j = 0; j < 2 * H; ++j) win[j] = win[j + R];
 #pragma unroll
         for (int r = 0; r < R; ++r) {
-            win[2 * H + r] = nwin[r];
-            hal[r] = nhal[r];
-            p0r[r] = np0[r];
-            vr[r] = nv[r];
+            const int za = za0 + r;
+            if (za >= za_end) break;
+            // ---- step A at row za (every lane) ----
+            float zA[2 * H + 1];
+#pragma unroll
+            for (int m = 0; m <= 2 * H; ++m) zA[m] = w1[r + m];
+            row1[li] = zA[H];
+            if (halo_lane) row1[hslot] = hal[r];
+            __builtin_amdgcn_wave_barrier();
+            float valA = 0.f;
+            if (za >= 0 && za < ntz) {
+                const bool row_inner = za >= pml && za < ntz - pml;
+                const bool rec_row = za == u.r_iz;
+                const bool src_row = za == u.s_iz;
+                const float* sp = row1 + li;
+                if (strip_inside && row_inner) {
+                    valA = point_value<LC, true>(u, cA, ix, za, zA, sp, p0r[r], vh[H + r], src_row, rec_row);
+                } else if (lane_ok) {
+                    valA = point_value<LC, false>(u, cA, ix, za, zA, sp, p0r[r], vh[H + r], src_row, rec_row);
+                }
+                if (owned && za >= zb && za < ze)
+                    store_point(u, cA, valA, p0r[r], zA[H], vh[H + r], ix, vo, row_off(za), (za - pml) * u.nx * 4,
+                                rg, rec_row, row_inner && inner_x);
+            }
+            w2[2 * H + r] = valA;
+            __builtin_amdgcn_wave_barrier();
+            // ---- step B at row zbr = za - H ----
+            const int zbr = za - H;
+            if (pair && zbr >= zb) {
+                float zB[2 * H + 1];
+#pragma unroll
+                for (int m = 0; m <= 2 * H; ++m) zB[m] = w2[r + m];
+                row2[li] = zB[H];
+                __builtin_amdgcn_wave_barrier();
+                const bool row_inner = zbr >= pml && zbr < ntz - pml;
+                const bool rec_row = zbr == u.r_iz;
+                const bool src_row = zbr == u.s_iz;
+                const float* sp = row2 + li;
+                const float p0b = w1[r];   // step A's p1 at row zbr
+                const float vb = vh[r];
+                float valB = 0.f;
+                if (strip_inside && row_inner) {
+                    valB = point_value<LC, true>(u, cB, ix, zbr, zB, sp, p0b, vb, src_row, rec_row);
+                } else if (lane_ok) {
+                    valB = point_value<LC, false>(u, cB, ix, zbr, zB, sp, p0b, vb, src_row, rec_row);
+                }
+                if (owned)
+                    store_point(u, cB, valB, p0b, zB[H], vb, ix, vo, row_off(zbr), (zbr - pml) * u.nx * 4, rg,
+                                rec_row, row_inner && inner_x);
+                __builtin_amdgcn_wave_barrier();
+            }
+        }
+
+        // Slide the windows to the next block.
+#pragma unroll
+        for (int j = 0; j < 2 * H; ++j) {
+            w1[j] = w1[j + R];
+            w2[j] = w2[j + R];
+        }
+#pragma unroll
+        for (int j = 0; j < H; ++j) vh[j] = vh[j + R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            w1[2 * H + r] = n1[r];
+            w2[2 * H + r] = 0.f;
+            hal[r] = nh[r];
+            p0r[r] = n0[r];
+            vh[H + r] = nv[r];
         }
     }
 }

@@ -102,12 +102,17 @@ int gpvi_fwi2d_batch(const gpvi_fwi2d_params *p, int nsample, const float *vel_i
                      int run_fwi, int ckpt_interval, int phase, int flags, void *workspace,
                      size_t workspace_bytes, void *stream);
 
-/* Launch shape of the multi-unit step: tile variant (0..3), persistent grid
+/* Launch shape of the multi-unit step: tile variant (0..5), persistent grid
  * (a multiple of 8; 0 = automatic) and steps per graph segment.  Process-wide
  * defaults; the autotuner sets them. */
 int gpvi_fwi2d_set_tuning(int variant, int grid, int graph_steps);
 int gpvi_fwi2d_get_tuning(int *variant, int *grid, int *graph_steps);
 int gpvi_fwi2d_tile_shape(int variant, int *tile_x, int *tile_z);
+/* Time steps per multi-unit launch (1 or 2; default 1).  With 2, each
+ * launch advances every unit by two consecutive steps of its pass, the
+ * second computed from the first in registers/LDS (temporal blocking). */
+int gpvi_fwi2d_set_time_blocking(int steps_per_launch);
+int gpvi_fwi2d_get_time_blocking(void);
 
 /* ---- Engine: continuous batching of misfit+gradient jobs ----
  * Jobs of any grid size join the running set at a step boundary; every step
