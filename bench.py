@@ -5,14 +5,15 @@ One "step" = one batched misfit+gradient evaluation of the c1 workload:
 16 velocity samples (mean-field Gaussian, std 50 m/s around the smoothed
 true model) x 20 shots on a 200x400 grid (228x428 padded), 200 receivers,
 8 Hz Ricker, dt = 1 ms, 2000 steps, nt_interval = 2, fp32, every
-(sample, shot) unit advanced by one launch per time step, replayed as HIP
-graph segments.  Inputs (models, observed data) are resident in HBM before
-the timed region.
+(sample, shot) unit advanced by each step launch, replayed as HIP graph
+segments.  Inputs (models, observed data) are resident in HBM before the
+timed region.  The launch shape is chosen by the autotuner (untimed).
 
 Printed (one JSON line): batched samples/s (``value``), the plain
 one-sample-per-launch GPU solver's samples/s on the same workload and the
 ratio, the gradient agreement of the two, the roofline of the dominant
-kernel (the multi-unit step), and the CPU oracle timed on this host.
+kernel (the multi-unit step), a served request stream (continuous
+batching; p50/p99 latency) and the CPU oracle timed on this host.
 """
 from __future__ import annotations
 
@@ -32,6 +33,10 @@ METRIC = "misfit+grad samples/sec batched vs plain GPU @1e-4 grad tol; HBM GB/s 
 HBM_PEAK_GBS = 8000.0
 
 
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -42,6 +47,12 @@ def parse():
     ap.add_argument("--plain-samples", type=int, default=1)
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--serve-requests", type=int, default=200)
+    ap.add_argument("--serve-rate", type=float, default=50.0, help="Poisson arrival rate (requests/s)")
+    ap.add_argument("--serve-low-rate", type=float, default=10.0, help="sub-saturation run rate (0 = skip)")
+    ap.add_argument("--serve-low-requests", type=int, default=60)
     ap.add_argument("--cpu-shots", type=int, default=4)
     ap.add_argument("--seed", type=int, default=1)
     return ap.parse_args()
@@ -58,6 +69,52 @@ def algorithmic_bytes_per_unit(p) -> int:
     return p.nt * p.ntp * 32 + nsave * p.ni * 16 + p.nt * p.nr * 12
 
 
+def serve_stream(args, rate, nreq, seed, dev):
+    """A Poisson stream of misfit+gradient requests through the server.
+
+    Requests follow the serving generator's "bench" mix (grids 100x200 and
+    200x400, 4-16 samples, 5-10 shots, 500-1000 steps; velocity samples
+    around a smoothed model of the c0 generator at that size, observed
+    data modelled from the unsmoothed model, both prepared before the
+    stream starts).  Latency runs from submit() to the result being ready."""
+    import torch
+    from geopvi_amd import synth
+    from geopvi_amd.forward.fwi2d import solver
+    from geopvi_amd.serving import FWIServer
+
+    rng = np.random.default_rng(seed)
+    reqs = []
+    for i in range(nreq):
+        p, m = synth.serving_request(rng, mix="bench")
+        vt = synth.gradient_anomaly_model(p.nz, p.nx, p.dx, p.dz, seed=seed + i)
+        vel = synth.gaussian_samples(synth.smooth(vt, 8.0), 50.0, m, seed=seed + 1000 + i).astype(np.float32)
+        obs = solver.forward_batch(p, torch.from_numpy(vt.astype(np.float32).reshape(1, -1)).to(dev))[0]
+        reqs.append((p, torch.from_numpy(vel).to(dev), obs))
+    gaps = rng.exponential(1.0 / rate, size=nreq)
+    torch.cuda.synchronize()
+    total_bytes = sum(m.shape[0] * p.ns * algorithmic_bytes_per_unit(p) for p, m, _ in reqs)
+    total_samples = sum(m.shape[0] for _, m, _ in reqs)
+    with FWIServer(dev.index or 0) as srv:
+        t0 = time.perf_counter()
+        t_next = t0
+        futs = []
+        for (p, vel, obs), gap in zip(reqs, gaps):
+            t_next += gap
+            delay = t_next - time.perf_counter()
+            if delay > 0:
+                time.sleep(delay)
+            futs.append(srv.submit(p, vel, obs))
+        res = [f.result() for f in futs]
+        t_end = max(r.finished for r in res)
+    lat = np.array([r.latency for r in res])
+    span = t_end - t0
+    return {"requests": nreq, "rate_req_s": rate, "p50_ms": round(float(np.percentile(lat, 50)) * 1e3, 1),
+            "p99_ms": round(float(np.percentile(lat, 99)) * 1e3, 1),
+            "mean_ms": round(float(lat.mean()) * 1e3, 1), "makespan_s": round(span, 3),
+            "samples_per_s": round(total_samples / span, 3),
+            "achieved_GBs": round(total_bytes / span / 1e9, 1)}
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -66,7 +123,7 @@ def main():
                           "n_gpus": args.gpus}))
         sys.exit(2)
     import torch
-    from geopvi_amd import synth, _lib
+    from geopvi_amd import _lib, autotune, synth
     from geopvi_amd.forward.fwi2d import solver
 
     _lib.load()
@@ -81,6 +138,15 @@ def main():
     obs_clean = solver.forward_batch(p, torch.from_numpy(vtrue.astype(np.float32).reshape(1, -1)).to(dev))[0]
     obs_np = synth.add_noise(obs_clean.cpu().numpy(), 0.01, seed=rng_seed + 200)
     obs = torch.from_numpy(obs_np).to(dev)
+    del obs_clean
+
+    log("inputs ready")
+    tune = None
+    if not args.no_autotune:
+        tune = autotune.autotune(p, args.nsample, cache_path=None)
+    variant, grid, graph_steps = _lib.get_tuning()
+    tb = _lib.get_time_blocking()
+    log(f"launch shape: variant {variant} grid {grid} steps/launch {tb}")
 
     wsb = solver.batch_workspace_bytes(p, args.nsample, args.ckpt)
     ws = torch.empty(wsb, dtype=torch.uint8, device=dev)
@@ -92,6 +158,8 @@ def main():
     for _ in range(args.warmup):
         run_batch()
     torch.cuda.synchronize()
+    # Events on torch's current stream: the stream every kernel of the
+    # batched evaluation (graph segments included) is enqueued on.
     ev0 = torch.cuda.Event(enable_timing=True)
     ev1 = torch.cuda.Event(enable_timing=True)
     t0 = time.perf_counter()
@@ -102,14 +170,19 @@ def main():
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     gpu_ms = ev0.elapsed_time(ev1)
+    log(f"timed {args.steps} batched evaluations: {wall:.3f} s")
     ms_per_step = wall * 1e3 / args.steps
     value = args.nsample * args.steps / wall
 
-    # Roofline of the dominant kernel (multi-unit step): algorithmic bytes of
-    # one launch / average launch time.  Launches per batch: nt forward +
-    # nt adjoint (+ nt recompute with checkpoints).
+    # Roofline of the dominant kernel (multi-unit step): algorithmic bytes
+    # of one launch / average launch time.  The step launches are all but a
+    # few small set-up and reduction kernels of the timed region, so the
+    # region's GPU time over the launch count bounds the step kernel's
+    # average duration from above (the profile under profiles/ gives it
+    # exactly).
+    sch = solver.schedule(p, args.ckpt, tb)
     units = args.nsample * p.ns
-    launches = p.nt * (2 if args.ckpt <= 0 or args.ckpt >= p.nt else 3)
+    launches = sch["launches"]
     bytes_per_launch = units * algorithmic_bytes_per_unit(p) / launches
     avg_launch_s = gpu_ms * 1e-3 / (args.steps * launches)
     achieved = bytes_per_launch / avg_launch_s / 1e9
@@ -117,7 +190,9 @@ def main():
     tpath = os.path.join(REPO, "profiles", "traffic_step_multi_c1.json")
     if os.path.exists(tpath):
         with open(tpath) as f:
-            traffic = json.load(f).get("hbm_bytes_per_launch")
+            tj = json.load(f)
+        if tj.get("tb") == tb and tj.get("variant") == variant and tj.get("ckpt") == args.ckpt:
+            traffic = tj.get("hbm_bytes_per_launch")
 
     out = {
         "metric": METRIC,
@@ -135,12 +210,16 @@ def main():
         "config": {"workload": "c1: 16 samples x 20 shots, 200x400 grid (228x428 padded), 200 receivers, "
                                "2000 steps, dt=1ms, nt_interval=2, fp32, batched forward+adjoint",
                    "nsample": args.nsample, "shots": p.ns, "nt": p.nt, "grid": [p.nz, p.nx],
-                   "ckpt_interval": args.ckpt},
+                   "ckpt_interval": args.ckpt, "tile_variant": variant, "launch_grid": grid,
+                   "steps_per_launch": tb, "graph_steps": graph_steps},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "kernel": "step_multi_kernel", "bytes_per_launch": int(bytes_per_launch),
-                     "avg_launch_us": round(avg_launch_s * 1e6, 3)},
+                     "launches_per_step": launches, "avg_launch_us": round(avg_launch_s * 1e6, 3)},
     }
+    if tune is not None:
+        out["autotune"] = {"us_per_time_step": tune["us_per_step"], "candidates": len(tune["timings"])}
+    del ws
 
     if not args.no_plain:
         vs0 = vel[0].contiguous()
@@ -152,6 +231,7 @@ def main():
         torch.cuda.synchronize()
         tp = time.perf_counter() - t0
         plain_sps = args.plain_samples / tp
+        log(f"plain solver: {tp:.3f} s for {args.plain_samples} sample(s)")
         gb = grad_b[(args.plain_samples - 1) % args.nsample].double()
         gp = grad_p.double()
         out["plain_samples_per_s"] = round(plain_sps, 4)
@@ -160,6 +240,17 @@ def main():
         mp = 0.5 * float(((syn_p.double() - obs.double()) ** 2).sum())
         mb = float(mis_b[(args.plain_samples - 1) % args.nsample])
         out["misfit_rel_batched_vs_plain"] = abs(mb - mp) / abs(mp)
+    del syn_b, grad_b
+    torch.cuda.empty_cache()
+
+    if not args.no_serve:
+        serve = {"mix": "bench (grids 100x200/200x400, 4-16 samples, 5-10 shots, 500-1000 steps)"}
+        serve["saturated"] = serve_stream(args, args.serve_rate, args.serve_requests, rng_seed + 300, dev)
+        log(f"served stream at {args.serve_rate}/s done")
+        if args.serve_low_rate > 0:
+            serve["sub_saturation"] = serve_stream(args, args.serve_low_rate, args.serve_low_requests,
+                                                   rng_seed + 400, dev)
+        out["serving"] = serve
 
     if not args.no_cpu:
         import oracle
@@ -170,6 +261,7 @@ def main():
         t0 = time.perf_counter()
         oracle.fwi(p, v0, obs_np, shots=(0, shots))
         tc = time.perf_counter() - t0
+        log(f"cpu oracle: {tc:.2f} s")
         per_sample = tc * p.ns / shots
         out["cpu_baseline"] = {"value": round(1.0 / per_sample, 6), "unit": "samples/s", "cores": ncores,
                                "kind": "port",

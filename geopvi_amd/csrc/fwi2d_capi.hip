@@ -12,6 +12,7 @@
 #include <mutex>
 #include <string>
 #include <tuple>
+#include <tuple>
 #include <vector>
 
 #include "fwi2d_kernels.h"
@@ -352,31 +353,57 @@ int job_setup(hipStream_t st, const gpvi_fwi2d_params* p, const Geom& g, const f
     return 0;
 }
 
-// Workgroups of the multi-unit step: one wavefront per work item up to
-// 8 waves per SIMD on 256 CUs (2048 workgroups of 4 waves), a multiple of 8.
-int auto_grid(int nitems)
+// Workgroups of the multi-unit step resident at once on the current device
+// (a multiple of 8), cached per (device, half-width, steps per launch).
+int step_capacity(int lc, int tb)
 {
-    Tuning t = tuning();
-    if (t.grid > 0) return std::max(8, t.grid / 8 * 8);
+    static std::mutex mu;
+    static std::map<std::tuple<int, int, int>, int> cache;
+    int dev = 0;
+    (void)hipGetDevice(&dev);
+    const auto key = std::make_tuple(dev, lc, tb);
+    std::lock_guard<std::mutex> lk(mu);
+    auto it = cache.find(key);
+    if (it != cache.end()) return it->second;
+    int cap = multi_step_capacity(lc, tb) / 8 * 8;
+    if (cap < 8) cap = 2048;  // occupancy query unavailable: 8 waves per SIMD on 256 CUs
+    cache[key] = cap;
+    return cap;
+}
+
+// Workgroups of one multi-unit launch (a multiple of 8): the tuned grid if
+// one is set, else one wavefront per work item, so the hardware dispatcher
+// hands the next item to whichever SIMD frees up first (items differ in cost:
+// absorbing bands, receiver rows, idle units).  nitems <= 0 (a graph that
+// must serve any item count): twice the resident grid, each wave walking
+// items until the list is done.
+int launch_grid(int tuned, int nitems, int lc, int tb)
+{
+    if (tuned > 0) return std::max(8, tuned / 8 * 8);
+    if (nitems <= 0) return 2 * step_capacity(lc, tb);
     const int waves_per_wg = kMultiThreads / 64;
     int g = (nitems + waves_per_wg - 1) / waves_per_wg;
     g = (g + 7) / 8 * 8;
-    return std::max(8, std::min(g, 2048));
+    return std::max(8, std::min(g, 1 << 20));
 }
 
 // ---------------------------------------------------------------------------
 // Graph segments: `steps` multi-unit step launches (one per half-width group
 // present) followed by the step-counter advance, built by capture on a
 // private stream.  Keyed by everything the captured launches bake in.
+// grid 0: each half-width group's launch uses launch_grid(0, nitems, ...)
+// with nitems the group's item count in `items` (0: the resident grid).
 struct GraphKey {
     const void* units;
     const void* tiles;
     const void* ctl;
-    int grid, variant, lc_mask, steps, device;
+    int grid, tb, lc_mask, steps, device;
+    int items[4];
     bool operator<(const GraphKey& o) const
     {
-        return std::tie(units, tiles, ctl, grid, variant, lc_mask, steps, device) <
-               std::tie(o.units, o.tiles, o.ctl, o.grid, o.variant, o.lc_mask, o.steps, o.device);
+        return std::tie(units, tiles, ctl, grid, tb, lc_mask, steps, device, items[0], items[1], items[2],
+                        items[3]) < std::tie(o.units, o.tiles, o.ctl, o.grid, o.tb, o.lc_mask, o.steps, o.device,
+                                             o.items[0], o.items[1], o.items[2], o.items[3]);
     }
 };
 
@@ -411,8 +438,9 @@ int get_graph(const GraphKey& key, hipGraphExec_t* out)
     for (int i = 0; i < key.steps && err == hipSuccess; ++i)
         for (int lc = 4; lc <= 7 && err == hipSuccess; ++lc)
             if (key.lc_mask & (1 << (lc - 4)))
-                err = launch_step_multi(key.variant, lc, key.grid, cap, (const UnitDesc*)key.units,
-                                        (const TileRef*)key.tiles, (const StepCtl*)key.ctl, i);
+                err = launch_step_multi(lc, key.tb, launch_grid(key.grid, key.items[lc - 4], lc, key.tb), cap,
+                                        (const UnitDesc*)key.units, (const TileRef*)key.tiles,
+                                        (const StepCtl*)key.ctl, i);
     if (err == hipSuccess) err = launch_advance(cap, (StepCtl*)key.ctl, key.steps);
     hipGraph_t graph = nullptr;
     hipError_t e2 = hipStreamEndCapture(cap, &graph);
@@ -653,13 +681,13 @@ int gpvi_fwi2d_batch(const gpvi_fwi2d_params* p, int nsample, const float* vel_i
     int n4[4] = {0, 0, 0, 0}, o4[4] = {0, 0, 0, 0};
     n4[g.lc - 4] = ntiles;
     HIP_TRY(launch_ctl_set(st, L.ctl, 0, n4, o4));
-    const int grid = auto_grid(ntiles);
+    const int grid = launch_grid(tn.grid, ntiles, g.lc, g.tb);
     if (flags & GPVI_FLAG_NO_GRAPH) {
         for (int s = 0; s < j.total; ++s)
-            HIP_TRY(launch_step_multi(g.variant, g.lc, grid, st, L.units, L.tiles, L.ctl, s));
+            HIP_TRY(launch_step_multi(g.lc, g.tb, grid, st, L.units, L.tiles, L.ctl, s));
     } else {
         const int seg = std::min(tn.graph_steps, j.total);
-        GraphKey key{L.units, L.tiles, L.ctl, grid, g.variant, 1 << (g.lc - 4), seg, current_device()};
+        GraphKey key{L.units, L.tiles, L.ctl, grid, g.tb, 1 << (g.lc - 4), seg, current_device(), {0, 0, 0, 0}};
         hipGraphExec_t exec;
         if ((rc = get_graph(key, &exec))) return rc;
         const int reps = (j.total + seg - 1) / seg;
@@ -783,7 +811,7 @@ int gpvi_engine_create(void* stream, int max_units, int max_tiles, gpvi_engine**
     e->variant = t.variant;
     e->tb = t.tb;
     e->seg_steps = t.graph_steps;
-    e->grid = t.grid > 0 ? std::max(8, t.grid / 8 * 8) : 2048;
+    e->grid = t.grid > 0 ? std::max(8, t.grid / 8 * 8) : 0;
     e->gstep = 0;
     e->next_id = 1;
     e->dirty = true;
@@ -889,7 +917,9 @@ int gpvi_engine_advance(gpvi_engine* e, int nsteps)
     if (e->active.empty() || nsteps <= 0) return 0;
     int mask = 0;
     for (const auto& jb : e->active) mask |= 1 << (jb.g.lc - 4);
-    GraphKey key{e->units, e->tiles, e->ctl, e->grid, e->variant, mask, e->seg_steps, e->device};
+    // Items 0: every launch uses the resident grid, so the graph survives
+    // changes of the job set.
+    GraphKey key{e->units, e->tiles, e->ctl, e->grid, e->tb, mask, e->seg_steps, e->device, {0, 0, 0, 0}};
     hipGraphExec_t exec;
     if ((rc = get_graph(key, &exec))) return rc;
     long long pending = 0;  // never run past the last active job's last step

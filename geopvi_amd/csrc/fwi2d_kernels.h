@@ -51,8 +51,9 @@ constexpr int kMultiThreads = 256;   // workgroup of the multi-unit step (4 inde
 constexpr int kSingleThreads = 64;   // workgroup of the single-unit step (1 wave)
 
 hipError_t launch_step_single(int variant, int lc, int grid, hipStream_t st, const UnitDesc& u, int s);
-hipError_t launch_step_multi(int variant, int lc, int grid, hipStream_t st, const UnitDesc* units,
-                             const TileRef* tiles, const StepCtl* ctl, int step_off);
+hipError_t launch_step_multi(int lc, int tb, int grid, hipStream_t st, const UnitDesc* units, const TileRef* tiles,
+                             const StepCtl* ctl, int step_off);
+int multi_step_capacity(int lc, int tb);
 hipError_t launch_advance(hipStream_t st, StepCtl* ctl, int n);
 hipError_t launch_ctl_set(hipStream_t st, StepCtl* ctl, int step, const int* ntiles4, const int* off4);
 hipError_t launch_zero(hipStream_t st, float* p, int64_t n);

@@ -8,32 +8,35 @@ namespace gpvi {
 // ---------------------------------------------------------------------------
 // Step kernels
 
-// One unit (the plain per-shot solver): one wavefront per strip segment.
+// One unit (the plain per-shot solver): one wavefront per strip segment,
+// one time step per launch.
 template <int LC, int NTH>
 __global__ __launch_bounds__(NTH) void step_single_kernel(UnitDesc u, int s)
 {
     constexpr int NW = NTH / 64;
-    __shared__ float rows[NW][2 * (kStripWidth + 2 * (LC - 1))];
+    __shared__ float rows[NW][kStripWidth + 2 * (LC - 1)];
     // readfirstlane: the compiler then knows the wave index is uniform, so
     // everything derived from it lives in scalar registers.
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int item = blockIdx.x * NW + wave;
-    if (item < u.tiles_x * u.tiles_z) process_item<LC>(u, s, item, rows[wave]);
+    if (item < u.tiles_x * u.tiles_z) process_item<LC, 1>(u, s, item, rows[wave]);
 }
 
-// Many units per launch.  A fixed grid of independent wavefronts walks the
-// work-item list (strip segments of every unit); the list is split into 8
-// contiguous chunks, one per XCD (workgroups are dispatched to XCDs
-// round-robin), so the rows a segment shares with its neighbours tend to be
-// read through the same L2.  Step and item count come from device memory so
-// a captured graph segment can be replayed without re-instantiation.
-template <int LC, int NTH>
+// Many units per launch, TB time steps per launch.  A grid of independent
+// wavefronts (sized to what is resident at once, so no workgroup waits for
+// another to finish) walks the work-item list (strip segments of every
+// unit); the list is split into 8 contiguous chunks, one per XCD
+// (workgroups are dispatched to XCDs round-robin), so the rows a segment
+// shares with its neighbours tend to be read through the same L2.  Step and
+// item count come from device memory so a captured graph segment can be
+// replayed without re-instantiation.
+template <int LC, int NTH, int TB>
 __global__ __launch_bounds__(NTH) void step_multi_kernel(const UnitDesc* __restrict__ units,
                                                          const TileRef* __restrict__ tiles,
                                                          const StepCtl* __restrict__ ctl, int step_off)
 {
     constexpr int NW = NTH / 64;
-    __shared__ float rows[NW][2 * (kStripWidth + 2 * (LC - 1))];
+    __shared__ float rows[NW][TB * (kStripWidth + 2 * (LC - 1))];
     // readfirstlane: the wave index, the work item and the unit descriptor
     // address are then known to be uniform (scalar loads, scalar registers).
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -53,7 +56,7 @@ __global__ __launch_bounds__(NTH) void step_multi_kernel(const UnitDesc* __restr
         const UnitDesc& u = units[tr.unit];
         const int s = gstep - u.start + u.s_off;
         if (s < u.s_off || s >= u.s_off + u.total) continue;
-        process_item<LC>(u, s, tr.tile, rows[wave]);
+        process_item<LC, TB>(u, s, tr.tile, rows[wave]);
     }
 }
 
@@ -242,27 +245,62 @@ hipError_t launch_step_single(int variant, int lc, int grid, hipStream_t st, con
     return hipGetLastError();
 }
 
-template <int LC>
+template <int LC, int TB>
 static hipError_t launch_multi_lc(int grid, hipStream_t st, const UnitDesc* units, const TileRef* tiles,
                                   const StepCtl* ctl, int step_off)
 {
-    hipLaunchKernelGGL((step_multi_kernel<LC, kMultiThreads>), dim3(grid), dim3(kMultiThreads), 0, st, units,
-                       tiles, ctl, step_off);
+    hipLaunchKernelGGL((step_multi_kernel<LC, kMultiThreads, TB>), dim3(grid), dim3(kMultiThreads), 0, st,
+                       units, tiles, ctl, step_off);
     return hipGetLastError();
 }
 
 // `grid` = workgroups of kMultiThreads threads, a multiple of 8.
-hipError_t launch_step_multi(int variant, int lc, int grid, hipStream_t st, const UnitDesc* units,
-                             const TileRef* tiles, const StepCtl* ctl, int step_off)
+hipError_t launch_step_multi(int lc, int tb, int grid, hipStream_t st, const UnitDesc* units, const TileRef* tiles,
+                             const StepCtl* ctl, int step_off)
 {
-    (void)variant;
-    switch (lc) {
-    case 4: return launch_multi_lc<4>(grid, st, units, tiles, ctl, step_off);
-    case 5: return launch_multi_lc<5>(grid, st, units, tiles, ctl, step_off);
-    case 6: return launch_multi_lc<6>(grid, st, units, tiles, ctl, step_off);
-    case 7: return launch_multi_lc<7>(grid, st, units, tiles, ctl, step_off);
+    switch (lc * 2 + (tb == 2)) {
+    case 8: return launch_multi_lc<4, 1>(grid, st, units, tiles, ctl, step_off);
+    case 9: return launch_multi_lc<4, 2>(grid, st, units, tiles, ctl, step_off);
+    case 10: return launch_multi_lc<5, 1>(grid, st, units, tiles, ctl, step_off);
+    case 11: return launch_multi_lc<5, 2>(grid, st, units, tiles, ctl, step_off);
+    case 12: return launch_multi_lc<6, 1>(grid, st, units, tiles, ctl, step_off);
+    case 13: return launch_multi_lc<6, 2>(grid, st, units, tiles, ctl, step_off);
+    case 14: return launch_multi_lc<7, 1>(grid, st, units, tiles, ctl, step_off);
+    case 15: return launch_multi_lc<7, 2>(grid, st, units, tiles, ctl, step_off);
     default: return hipErrorInvalidValue;
     }
+}
+
+template <int LC, int TB>
+static int resident_blocks_lc()
+{
+    int nb = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, step_multi_kernel<LC, kMultiThreads, TB>, kMultiThreads,
+                                                     0) != hipSuccess)
+        return 0;
+    return nb;
+}
+
+// Workgroups of the multi-unit step resident on the current device at once
+// (per-CU occupancy from the kernel's registers and LDS, times the CUs).
+int multi_step_capacity(int lc, int tb)
+{
+    int nb = 0;
+    switch (lc * 2 + (tb == 2)) {
+    case 8: nb = resident_blocks_lc<4, 1>(); break;
+    case 9: nb = resident_blocks_lc<4, 2>(); break;
+    case 10: nb = resident_blocks_lc<5, 1>(); break;
+    case 11: nb = resident_blocks_lc<5, 2>(); break;
+    case 12: nb = resident_blocks_lc<6, 1>(); break;
+    case 13: nb = resident_blocks_lc<6, 2>(); break;
+    case 14: nb = resident_blocks_lc<7, 1>(); break;
+    case 15: nb = resident_blocks_lc<7, 2>(); break;
+    default: return 0;
+    }
+    int dev = 0, cus = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return 0;
+    return nb * cus;
 }
 
 hipError_t launch_advance(hipStream_t st, StepCtl* ctl, int n)

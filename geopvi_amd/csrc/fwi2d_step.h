@@ -42,11 +42,15 @@ __device__ __forceinline__ __attribute__((address_space(1))) T* G(T* p)
 // That holds for 2^-16 <= |b| <= 2^16 and a == 0 or 2^-80 <= |a| < 2^80;
 // other lanes take the IEEE division.  The host checks the divisor range and
 // verifies the identity over a full binade before it sets UnitDesc::fastdiv.
-__device__ __forceinline__ bool cdiv_in_range(float a)
+// `hi` is 2^80 when the job may use the identity and 0 when it may not (then
+// no value is in range and every lane divides).
+__device__ __forceinline__ bool cdiv_in_range(float a, float hi)
 {
     const float aa = __builtin_fabsf(a);
-    return (aa < 0x1p80f) && (aa >= 0x1p-80f || aa == 0.f);
+    return (aa < hi) && (aa >= 0x1p-80f || aa == 0.f);
 }
+
+__device__ __forceinline__ float div_hi(const UnitDesc& u) { return u.fastdiv ? 0x1p80f : 0.f; }
 
 __device__ __forceinline__ float cdiv_fast(float a, float b, float y)
 {
@@ -56,22 +60,26 @@ __device__ __forceinline__ float cdiv_fast(float a, float b, float y)
 }
 
 // (a / b) / b, as the reference writes d2p/dx/dx and (.)/dt/dt.
-__device__ __forceinline__ float div2_const(float a, float b, float y, bool fast)
+__device__ __forceinline__ float div2_const(float a, float b, float y, float hi)
 {
-    if (fast) {
-        const float q = cdiv_fast(cdiv_fast(a, b, y), b, y);
-        if (__builtin_expect(cdiv_in_range(a), 1)) return q;
-    }
+    const float q = cdiv_fast(cdiv_fast(a, b, y), b, y);
+    if (__builtin_expect(cdiv_in_range(a, hi), 1)) return q;
     return (a / b) / b;
 }
 
-__device__ __forceinline__ float div1_const(float a, float b, float y, bool fast)
+__device__ __forceinline__ float div1_const(float a, float b, float y, float hi)
 {
-    if (fast) {
-        const float q = cdiv_fast(a, b, y);
-        if (__builtin_expect(cdiv_in_range(a), 1)) return q;
-    }
+    const float q = cdiv_fast(a, b, y);
+    if (__builtin_expect(cdiv_in_range(a, hi), 1)) return q;
     return a / b;
+}
+
+// (a / bx) / bx + (c / bz) / bz with one range test for both quotients.
+__device__ __forceinline__ float lap_const(float a, float bx, float yx, float c, float bz, float yz, float hi)
+{
+    const float q = cdiv_fast(cdiv_fast(a, bx, yx), bx, yx) + cdiv_fast(cdiv_fast(c, bz, yz), bz, yz);
+    if (__builtin_expect(cdiv_in_range(a, hi) && cdiv_in_range(c, hi), 1)) return q;
+    return (a / bx) / bx + (c / bz) / bz;
 }
 
 // Order-2 transmitting extrapolation (fdtd2D_modelling.c:840-846): q1..q3
@@ -169,7 +177,7 @@ __device__ __forceinline__ float point_value(const UnitDesc& u, const StepCtx& c
                                              bool src_row, bool rec_row)
 {
     constexpr int H = LC - 1;
-    const bool fast = u.fastdiv != 0;
+    const float hi = div_hi(u);
     const float p1c = zn[H];
     const int ntx = u.ntx, ntz = u.ntz, pml = u.pml;
     float val = p1c;
@@ -182,7 +190,7 @@ __device__ __forceinline__ float point_value(const UnitDesc& u, const StepCtx& c
 #pragma unroll
         for (int m = 1; m < LC; ++m) dzz = dzz + u.fd[m] * (zn[H + m] + zn[H - m]);
         const float vdt = v * u.dt;
-        const float lap = div2_const(dxx, u.dx, u.rdx, fast) + div2_const(dzz, u.dz, u.rdz, fast);
+        const float lap = lap_const(dxx, u.dx, u.rdx, dzz, u.dz, u.rdz, hi);
         val = (2.f * p1c - p0c) + (vdt * vdt) * lap;
     }
     if (!INSIDE) {
@@ -193,7 +201,7 @@ __device__ __forceinline__ float point_value(const UnitDesc& u, const StepCtx& c
         const bool rig = ix >= ntx - pml && ix < ntx - LC;
         if (top || bot || lef || rig) {
             const int ip = iz * u.pitch + ix;
-            const float sc = div1_const(v * u.dt, u.dx, u.rdx, fast);
+            const float sc = div1_const(v * u.dt, u.dx, u.rdx, hi);
             const float t11 = ((2.f - sc) * (1.f - sc)) / 2.f;
             const float t12 = sc * (2.f - sc);
             const float t13 = (sc * (sc - 1.f)) / 2.f;
@@ -244,7 +252,7 @@ __device__ __forceinline__ void store_point(const UnitDesc& u, const StepCtx& c,
         }
         if (c.has_save && inner) {
             float sv = val;
-            if (u.run_fwi) sv = div2_const((val - 2.f * p1c) + p0c, dt, u.rdt, u.fastdiv != 0);
+            if (u.run_fwi) sv = div2_const((val - 2.f * p1c) + p0c, dt, u.rdt, div_hi(u));
             bstore(sv, c.save, gvo, go);
         }
         if (c.mode == kFwd && rec_row) {
@@ -279,8 +287,9 @@ __device__ __forceinline__ void store_point(const UnitDesc& u, const StepCtx& c,
 //   store; A's values on the other lanes and rows only feed B, and are
 //   recomputed bit-identically by the neighbouring strips and segments.
 //
-// `lds` is the wave's private scratch: two rows of 64 + 2H floats.
-template <int LC>
+// TB (1 or 2) is the unit's steps per macro step, UnitDesc::tb.  `lds` is
+// the wave's private scratch: TB rows of 64 + 2H floats.
+template <int LC, int TB>
 __device__ __forceinline__ void process_item(const UnitDesc& u, int s, int item, float* lds)
 {
     constexpr int H = LC - 1;
@@ -289,7 +298,7 @@ __device__ __forceinline__ void process_item(const UnitDesc& u, int s, int item,
     constexpr int RW = 64 + 2 * H;     // LDS row length
     const StepInfo si = decode_step(u, s);
     if (si.mode == kIdle) return;
-    const bool pair = si.count == 2;
+    const bool pair = TB == 2 && si.count == 2;
 
     const int ntx = u.ntx, ntz = u.ntz, pitch = u.pitch, pml = u.pml;
     const int64_t fe = u.field_elems;
@@ -314,7 +323,7 @@ __device__ __forceinline__ void process_item(const UnitDesc& u, int s, int item,
     const int zb = seg * u.zseg;
     const int ze = min(zb + u.zseg, ntz);
     const int lane = threadIdx.x & 63;
-    const int loff = (u.tb == 2) ? H : 0;         // first owned lane
+    const int loff = TB == 2 ? H : 0;             // first owned lane
     const int ix = x0 - loff + lane;              // this lane's column
     const bool lane_ok = ix >= 0 && ix < ntx;
     const bool owned = lane_ok && lane >= loff && lane < loff + u.stripw;
@@ -335,7 +344,7 @@ __device__ __forceinline__ void process_item(const UnitDesc& u, int s, int item,
     const Rsrc rv = make_rsrc(u.vel - margin);
     const Rsrc rg = make_rsrc(u.grad);
     float* row1 = lds;
-    float* row2 = lds + RW;
+    float* row2 = lds + (TB - 1) * RW;
 
     // Rows of step A: the segment, plus H on each side for a pair.
     const int za_begin = pair ? zb - H : zb;

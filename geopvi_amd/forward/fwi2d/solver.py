@@ -70,6 +70,31 @@ def batch_workspace_bytes(params: FWIParams, nsample: int, ckpt_interval: int = 
     return int(n)
 
 
+def schedule(params: FWIParams, ckpt_interval: int = 0, steps_per_launch: int = 1) -> dict:
+    """Adjoint schedule and launch counts of a batched evaluation.
+
+    Mirrors the library's schedule: ckpt_interval <= 0 or >= nt keeps the
+    full store (one segment); otherwise it is rounded up to a multiple of
+    nt_interval and the backward pass recomputes each segment from its
+    checkpoint before running its adjoint.  Each launch advances every unit
+    by `steps_per_launch` steps of one pass (a pass of odd length ends with
+    a one-step launch)."""
+    nt, k, tb = params.nt, params.nt_interval, int(steps_per_launch)
+    cdiv = lambda a, b: -(-a // b)  # noqa: E731
+    c = cdiv(ckpt_interval, k) * k if 0 < ckpt_interval < nt else nt
+    nseg = cdiv(nt, c)
+    if nseg <= 1:
+        nseg, c = 1, nt
+    last = nt - (nseg - 1) * c
+    fwd = cdiv(nt, tb)
+    if nseg == 1:
+        back = cdiv(nt, tb)
+    else:
+        back = (nseg - 1) * 2 * cdiv(c, tb) + 2 * cdiv(last, tb)
+    return {"nseg": nseg, "ckpt": c, "last_len": last, "launches_forward": fwd, "launches_backward": back,
+            "launches": fwd + back, "recomputed_steps": 0 if nseg == 1 else nt}
+
+
 # ---------------------------------------------------------------------------
 # Plain solver: one sample, shots in sequence, one launch per time step.
 

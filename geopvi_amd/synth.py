@@ -142,14 +142,24 @@ def true_model(name: str, seed: int = 0) -> np.ndarray:
     return gradient_anomaly_model(p.nz, p.nx, p.dx, p.dz, seed=seed)
 
 
-def serving_request(rng: np.random.Generator):
-    """One request of the serving mix: grid, samples, shots and steps drawn
-    uniformly from {100x200, 200x400, 300x600, 400x800}, 4..64, 5..40 and
-    1000..3000; receivers every 2 cells across half the width."""
-    nz, nx = [(100, 200), (200, 400), (300, 600), (400, 800)][int(rng.integers(0, 4))]
-    nsample = int(rng.integers(4, 65))
-    ns = int(rng.integers(5, 41))
-    nt = int(rng.integers(1000, 3001))
+SERVING_MIXES = {
+    # grids, nsample range, shot range, step range (inclusive)
+    "c3": ([(100, 200), (200, 400), (300, 600), (400, 800)], (4, 64), (5, 40), (1000, 3000)),
+    # the same generator at a scale whose 200-request stream takes seconds
+    "bench": ([(100, 200), (200, 400)], (4, 16), (5, 10), (500, 1000)),
+}
+
+
+def serving_request(rng: np.random.Generator, mix: str = "c3"):
+    """One request of a serving mix: grid, samples, shots and steps drawn
+    uniformly from the mix's sets (c3: {100x200, 200x400, 300x600,
+    400x800}, 4..64, 5..40 and 1000..3000); receivers every 2 cells across
+    half the width, shots spread evenly."""
+    grids, (m0, m1), (s0, s1), (t0, t1) = SERVING_MIXES[mix]
+    nz, nx = grids[int(rng.integers(0, len(grids)))]
+    nsample = int(rng.integers(m0, m1 + 1))
+    ns = int(rng.integers(s0, s1 + 1))
+    nt = int(rng.integers(t0, t1 + 1))
     ds = nx // ns
     kw = dict(_COMMON)
     kw.update(nz=nz, nx=nx, ns=ns, ds=ds, ns0=ds // 2, nr=nx // 2, dr=2, nr0=0, f0=10.0, nt=nt)

@@ -1,0 +1,82 @@
+"""Launch shapes never change results.
+
+Every tile variant, grid and steps-per-launch setting only changes which
+wavefront computes which points; records, gradients and misfits must be
+bit-identical across all of them and match the CPU oracle to the parity
+bars.  Also runs the autotuner on a small case.
+"""
+import numpy as np
+import pytest
+import torch
+
+from helpers import GRAD_RTOL, MISFIT_RTOL, case, rel_err, rel_l2
+from geopvi_amd import _lib, autotune
+from geopvi_amd.forward.fwi2d import solver
+
+pytestmark = pytest.mark.gpu
+REC_RTOL = 1e-6
+
+
+def dev(x):
+    return torch.from_numpy(np.ascontiguousarray(np.asarray(x, np.float32))).cuda()
+
+
+def host(t):
+    torch.cuda.synchronize()
+    return t.detach().cpu().numpy()
+
+
+@pytest.fixture(scope="module")
+def small(oracle_mod):
+    # odd nt: a two-step launch schedule ends every pass with a one-step macro
+    p, vt, vs = case("small", seed=11, nt=301, nt_interval=2)
+    obs = oracle_mod.forward(p, vt)
+    syn_o, grad_o = oracle_mod.fwi(p, vs, obs)
+    vel = np.stack([vs.reshape(-1), (0.5 * (vs + vt)).reshape(-1)])
+    return p, vel, obs, syn_o, grad_o
+
+
+@pytest.fixture()
+def restore_tuning():
+    saved = (_lib.get_tuning(), _lib.get_time_blocking())
+    yield
+    (v, g, s), tb = saved
+    _lib.set_tuning(v, g, s)
+    _lib.set_time_blocking(tb)
+
+
+@pytest.mark.parametrize("ckpt", [0, 60])
+def test_all_launch_shapes_bit_identical(small, oracle_mod, restore_tuning, ckpt):
+    p, vel, obs, syn_o, grad_o = small
+    vel_t, obs_t = dev(vel), dev(obs)
+    ref = None
+    for variant in range(6):
+        for grid in (0, 8, 64):
+            for tb in (1, 2):
+                for graph_steps in ((32,) if grid else (32, 5)):
+                    _lib.set_tuning(variant, grid, graph_steps)
+                    _lib.set_time_blocking(tb)
+                    out = [host(t) for t in solver.fwi_batch(p, vel_t, obs_t, ckpt_interval=ckpt)]
+                    tag = f"variant={variant} grid={grid} tb={tb} seg={graph_steps} ckpt={ckpt}"
+                    print(tag, flush=True)
+                    if ref is None:
+                        ref = out
+                        syn, grad, mis = out
+                        assert rel_l2(syn[0], syn_o) <= REC_RTOL
+                        assert rel_l2(grad[0], grad_o) <= GRAD_RTOL
+                        assert rel_err(mis[0], oracle_mod.misfit_f32(syn_o, obs)) <= MISFIT_RTOL
+                    else:
+                        for a, b in zip(out, ref):
+                            assert np.array_equal(a, b), tag
+
+
+def test_autotune_small(small, restore_tuning):
+    p, vel, obs, _, _ = small
+    cands = autotune.candidates(variants=(0, 4), grids=(0, 64), steps_per_launch=(1, 2))
+    res = autotune.autotune(p, 2, cands=cands, steps=40, cache_path=None)
+    assert (res["variant"], res["grid"], res["tb"]) in cands
+    assert res["us_per_step"] > 0 and not res["cached"]
+    assert len(res["timings"]) == len(cands)
+    # applied process-wide
+    v, g, _ = _lib.get_tuning()
+    assert (v, g, _lib.get_time_blocking()) == (res["variant"], res["grid"], res["tb"])
