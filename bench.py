@@ -49,11 +49,14 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--tune", default="", help="variant,grid,steps_per_launch: apply instead of autotuning")
+    ap.add_argument("--copy-calibration-mb", type=int, default=0,
+                    help="also run a plain device copy of this many MiB (HBM counter calibration)")
     ap.add_argument("--serve-requests", type=int, default=200)
     ap.add_argument("--serve-rate", type=float, default=50.0, help="Poisson arrival rate (requests/s)")
     ap.add_argument("--serve-low-rate", type=float, default=10.0, help="sub-saturation run rate (0 = skip)")
     ap.add_argument("--serve-low-requests", type=int, default=60)
-    ap.add_argument("--cpu-shots", type=int, default=4)
+    ap.add_argument("--cpu-shots", type=int, default=20)
     ap.add_argument("--seed", type=int, default=1)
     return ap.parse_args()
 
@@ -141,8 +144,20 @@ def main():
     del obs_clean
 
     log("inputs ready")
+    if args.copy_calibration_mb > 0:
+        # A kernel of known traffic (reads and writes N MiB once) for
+        # scaling the profiler's FETCH_SIZE / WRITE_SIZE counters.
+        n = args.copy_calibration_mb * (1 << 20) // 4
+        src = torch.ones(n, dtype=torch.float32, device=dev)
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     tune = None
-    if not args.no_autotune:
+    if args.tune:
+        autotune.apply(tuple(int(x) for x in args.tune.split(",")))
+    elif not args.no_autotune:
         tune = autotune.autotune(p, args.nsample, cache_path=None)
     variant, grid, graph_steps = _lib.get_tuning()
     tb = _lib.get_time_blocking()
