@@ -1,5 +1,5 @@
 // C ABI of the FWI-2D solver (declared in include/geopvi_fwi2d.h).
-#include "../../include/geopvi_fwi2d.h"
+#include "geopvi_fwi2d.h"
 
 #include <hip/hip_runtime.h>
 
@@ -61,24 +61,9 @@ Tuning tuning()
 int fd_coeffs(int lc, float* fd)
 {
     for (int i = 0; i < 8; ++i) fd[i] = 0.f;
-    switch (lc) {
-    case 4:
-        fd[0] = -49.0 / 18; fd[1] = 3.0 / 2; fd[2] = -3.0 / 20; fd[3] = 1.0 / 90;
-        return 0;
-    case 5:
-        fd[0] = -205.0 / 72; fd[1] = 8.0 / 5; fd[2] = -1.0 / 5; fd[3] = 8.0 / 315; fd[4] = -1.0 / 560;
-        return 0;
-    case 6:
-        fd[0] = -5269.0 / 1800; fd[1] = 5.0 / 3; fd[2] = -5.0 / 21; fd[3] = 5.0 / 126;
-        fd[4] = -5.0 / 1008; fd[5] = 1.0 / 3150;
-        return 0;
-    case 7:
-        fd[0] = -5369.0 / 1800; fd[1] = 12.0 / 7; fd[2] = -15.0 / 56; fd[3] = 10.0 / 189;
-        fd[4] = -1.0 / 112; fd[5] = 2.0 / 1925; fd[6] = -1.0 / 16632;
-        return 0;
-    default:
-        return -1;
-    }
+    if (lc < 4 || lc > 7) return -1;
+    for (int m = 0; m < lc; ++m) fd[m] = fd_weight(lc, m);
+    return 0;
 }
 
 int validate(const gpvi_fwi2d_params* p)

@@ -28,6 +28,30 @@ constexpr int kMaxLc = 7;       // solver half-width (configured Lc 3..6, plus o
 constexpr int kPadRows = 16;    // margin rows above and below every field buffer
 constexpr int kRowBlock = 4;    // rows per prefetch block of a strip segment
 
+// Central second-derivative weights for solver half-width lc = 4..7
+// (configured Lc 3..6), weight m = 0..lc-1: the exact rationals of the
+// reference's table (fdtd2D_modelling.c:1272-1326) rounded to float.  One
+// definition for the host (UnitDesc::fd, validation) and the kernels (where
+// lc is a template parameter, so every weight is a literal).
+__host__ __device__ constexpr float fd_weight(int lc, int m)
+{
+    switch (lc * 8 + m) {
+    case 32: return (float)(-49.0 / 18); case 33: return (float)(3.0 / 2);
+    case 34: return (float)(-3.0 / 20);  case 35: return (float)(1.0 / 90);
+    case 40: return (float)(-205.0 / 72); case 41: return (float)(8.0 / 5);
+    case 42: return (float)(-1.0 / 5);    case 43: return (float)(8.0 / 315);
+    case 44: return (float)(-1.0 / 560);
+    case 48: return (float)(-5269.0 / 1800); case 49: return (float)(5.0 / 3);
+    case 50: return (float)(-5.0 / 21);      case 51: return (float)(5.0 / 126);
+    case 52: return (float)(-5.0 / 1008);    case 53: return (float)(1.0 / 3150);
+    case 56: return (float)(-5369.0 / 1800); case 57: return (float)(12.0 / 7);
+    case 58: return (float)(-15.0 / 56);     case 59: return (float)(10.0 / 189);
+    case 60: return (float)(-1.0 / 112);     case 61: return (float)(2.0 / 1925);
+    case 62: return (float)(-1.0 / 16632);
+    default: return 0.f;
+    }
+}
+
 // Per-unit modes of one local step.
 enum StepMode : int { kFwd = 0, kRec = 1, kAdj = 2, kIdle = 3 };
 

@@ -47,10 +47,24 @@ __device__ __forceinline__ __attribute__((address_space(1))) T* G(T* p)
 __device__ __forceinline__ bool cdiv_in_range(float a, float hi)
 {
     const float aa = __builtin_fabsf(a);
-    return (aa < hi) && (aa >= 0x1p-80f || aa == 0.f);
+    return (aa < hi) & ((aa >= 0x1p-80f) | (aa == 0.f));
 }
 
 __device__ __forceinline__ float div_hi(const UnitDesc& u) { return u.fastdiv ? 0x1p80f : 0.f; }
+
+// Per-job constants of the point update, read from the unit descriptor once
+// per work item.  (Read inside the row loop, every field would be fetched
+// again after each store: the compiler cannot rule out that the buffer
+// stores alias the descriptor.)
+struct PointConsts {
+    float dx, dz, dt, rdx, rdz, rdt, hi;
+    int run_fwi, k;
+};
+
+__device__ __forceinline__ PointConsts point_consts(const UnitDesc& u)
+{
+    return PointConsts{u.dx, u.dz, u.dt, u.rdx, u.rdz, u.rdt, div_hi(u), u.run_fwi, u.k};
+}
 
 __device__ __forceinline__ float cdiv_fast(float a, float b, float y)
 {
@@ -78,35 +92,10 @@ __device__ __forceinline__ float div1_const(float a, float b, float y, float hi)
 __device__ __forceinline__ float lap_const(float a, float bx, float yx, float c, float bz, float yz, float hi)
 {
     const float q = cdiv_fast(cdiv_fast(a, bx, yx), bx, yx) + cdiv_fast(cdiv_fast(c, bz, yz), bz, yz);
-    if (__builtin_expect(cdiv_in_range(a, hi) && cdiv_in_range(c, hi), 1)) return q;
+    if (__builtin_expect(cdiv_in_range(a, hi) & cdiv_in_range(c, hi), 1)) return q;
     return (a / bx) / bx + (c / bz) / bz;
 }
 
-// Order-2 transmitting extrapolation (fdtd2D_modelling.c:840-846): q1..q3
-// are p1 at 0, 1, 2 cells inward, a0..a4 are p0 at 0..4 cells inward.
-__device__ __forceinline__ float abc_extrapolate(float q1, float q2, float q3, const float* p0, int ip, int d0,
-                                                 float t11, float t12, float t13)
-{
-    const auto g = G(p0);
-    const float a0 = g[ip], a1 = g[ip + d0], a2 = g[ip + 2 * d0], a3 = g[ip + 3 * d0], a4 = g[ip + 4 * d0];
-    float h = 2.f * t11 * q1;
-    h = h + 2.f * t12 * q2;
-    h = h + 2.f * t13 * q3;
-    h = h - t11 * t11 * a0;
-    h = h - 2.f * t11 * t12 * a1;
-    h = h - (t12 * t12 + 2.f * t11 * t13) * a2;
-    h = h - 2.f * t12 * t13 * a3;
-    h = h - t13 * t13 * a4;
-    return h;
-}
-
-// w*H + (1.0 - w)*U: float product, double blend, float store (fdtd.c:840-847).
-__device__ __forceinline__ float abc_blend(float w, float h, float u)
-{
-    return (float)((double)(w * h) + (1.0 - (double)w) * (double)u);
-}
-
-// ---------------------------------------------------------------------------
 // Raw buffer access: a 128-bit scalar resource (base address, byte range,
 // 32-bit data format) plus a per-lane byte offset (VGPR) and a uniform
 // per-row byte offset (SGPR).  One instruction per access, no per-lane
@@ -129,6 +118,34 @@ __device__ __forceinline__ void bstore(float v, Rsrc r, unsigned voff, int soff)
     __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned int, v), r, voff, soff, 0);
 }
 
+// Order-2 transmitting extrapolation (fdtd2D_modelling.c:840-846): q1..q3
+// are p1 at 0, 1, 2 cells inward, a0..a4 are p0 at 0..4 cells inward, read
+// through p0's resource at lane offset vo and row offsets ro + j*dstep
+// (dstep: the byte step one cell inward, +-4 along x or +-pitch*4 along z;
+// the margins keep every offset non-negative).
+__device__ __forceinline__ float abc_extrapolate(float q1, float q2, float q3, Rsrc p0, unsigned vo, int ro,
+                                                 int dstep, float t11, float t12, float t13)
+{
+    const float a0 = bload(p0, vo, ro), a1 = bload(p0, vo, ro + dstep), a2 = bload(p0, vo, ro + 2 * dstep);
+    const float a3 = bload(p0, vo, ro + 3 * dstep), a4 = bload(p0, vo, ro + 4 * dstep);
+    float h = 2.f * t11 * q1;
+    h = h + 2.f * t12 * q2;
+    h = h + 2.f * t13 * q3;
+    h = h - t11 * t11 * a0;
+    h = h - 2.f * t11 * t12 * a1;
+    h = h - (t12 * t12 + 2.f * t11 * t13) * a2;
+    h = h - 2.f * t12 * t13 * a3;
+    h = h - t13 * t13 * a4;
+    return h;
+}
+
+// w*H + (1.0 - w)*U: float product, double blend, float store (fdtd.c:840-847).
+__device__ __forceinline__ float abc_blend(float w, float h, float u)
+{
+    return (float)((double)(w * h) + (1.0 - (double)w) * (double)u);
+}
+
+// ---------------------------------------------------------------------------
 // ---------------------------------------------------------------------------
 // One time step of a macro, as seen by the points a wave updates.
 struct StepCtx {
@@ -136,12 +153,11 @@ struct StepCtx {
     bool first_image;  // first imaging step of the adjoint pass: the gradient starts from 0
     bool has_save;     // wavefield store (forward-like) or imaging (adjoint) at this step
     bool has_ck;       // checkpoint (p0, p1) written at this step
-    const float* p0g;  // p0 of this step in global memory (inward reads of the ABC)
     Rsrc out, save, ck0, ck1;
 };
 
-__device__ __forceinline__ StepCtx make_step(const UnitDesc& u, int mode, int it, int seg, const float* p0g,
-                                             float* out, int64_t margin)
+__device__ __forceinline__ StepCtx make_step(const UnitDesc& u, int mode, int it, int seg, float* out,
+                                             int64_t margin)
 {
     StepCtx c;
     c.it = it;
@@ -162,7 +178,6 @@ __device__ __forceinline__ StepCtx make_step(const UnitDesc& u, int mode, int it
         c.ck1 = make_rsrc(u.ck + (int64_t)(2 * j + 1) * u.field_elems - margin);
     }
     c.first_image = u.grad_init && (it == ((u.nt - 1) / u.k) * u.k);
-    c.p0g = p0g;
     c.out = make_rsrc(out - margin);
     return c;
 }
@@ -171,26 +186,28 @@ __device__ __forceinline__ StepCtx make_step(const UnitDesc& u, int mode, int it
 // residual injection.  zn[H + m] is p1 at row iz + m (|m| <= H, this lane's
 // column), sp[m] p1 at column ix + m of row iz (the LDS row).  INSIDE: every
 // lane of the wave is an inner-grid point away from the absorbing bands.
+// rp0/vo/ro: p0 of this step's resource with this lane's and this row's
+// byte offsets (inward reads of the ABC).
 template <int LC, bool INSIDE>
-__device__ __forceinline__ float point_value(const UnitDesc& u, const StepCtx& c, int ix, int iz,
+__device__ __forceinline__ float point_value(const UnitDesc& u, const PointConsts& kc, const StepCtx& c, int ix, int iz,
                                              const float (&zn)[2 * LC - 1], const float* sp, float p0c, float v,
-                                             bool src_row, bool rec_row)
+                                             bool src_row, bool rec_row, Rsrc rp0, unsigned vo, int ro)
 {
     constexpr int H = LC - 1;
-    const float hi = div_hi(u);
+    const float hi = kc.hi;
     const float p1c = zn[H];
     const int ntx = u.ntx, ntz = u.ntz, pml = u.pml;
     float val = p1c;
     if (INSIDE || (iz >= LC && iz < ntz - LC && ix >= LC && ix < ntx - LC)) {
         // S1: (2*p1 - p0) + (v*dt)^2 * (d2x/dx/dx + d2z/dz/dz)
-        float dxx = u.fd[0] * p1c;
+        float dxx = fd_weight(LC, 0) * p1c;
 #pragma unroll
-        for (int m = 1; m < LC; ++m) dxx = dxx + u.fd[m] * (sp[m] + sp[-m]);
-        float dzz = u.fd[0] * p1c;
+        for (int m = 1; m < LC; ++m) dxx = dxx + fd_weight(LC, m) * (sp[m] + sp[-m]);
+        float dzz = fd_weight(LC, 0) * p1c;
 #pragma unroll
-        for (int m = 1; m < LC; ++m) dzz = dzz + u.fd[m] * (zn[H + m] + zn[H - m]);
-        const float vdt = v * u.dt;
-        const float lap = lap_const(dxx, u.dx, u.rdx, dzz, u.dz, u.rdz, hi);
+        for (int m = 1; m < LC; ++m) dzz = dzz + fd_weight(LC, m) * (zn[H + m] + zn[H - m]);
+        const float vdt = v * kc.dt;
+        const float lap = lap_const(dxx, kc.dx, kc.rdx, dzz, kc.dz, kc.rdz, hi);
         val = (2.f * p1c - p0c) + (vdt * vdt) * lap;
     }
     if (!INSIDE) {
@@ -200,26 +217,26 @@ __device__ __forceinline__ float point_value(const UnitDesc& u, const StepCtx& c
         const bool lef = ix >= LC && ix < pml;
         const bool rig = ix >= ntx - pml && ix < ntx - LC;
         if (top || bot || lef || rig) {
-            const int ip = iz * u.pitch + ix;
-            const float sc = div1_const(v * u.dt, u.dx, u.rdx, hi);
+            const int pstep = u.pitch * 4;
+            const float sc = div1_const(v * kc.dt, kc.dx, kc.rdx, hi);
             const float t11 = ((2.f - sc) * (1.f - sc)) / 2.f;
             const float t12 = sc * (2.f - sc);
             const float t13 = (sc * (sc - 1.f)) / 2.f;
-            const auto wt = G(u.wtab);
+            const Rsrc rw = make_rsrc(u.wtab);
             if (top)
-                val = abc_blend(wt[iz - LC],
-                                abc_extrapolate(p1c, zn[H + 1], zn[H + 2], c.p0g, ip, u.pitch, t11, t12, t13), val);
+                val = abc_blend(bload(rw, 0u, (iz - LC) * 4),
+                                abc_extrapolate(p1c, zn[H + 1], zn[H + 2], rp0, vo, ro, pstep, t11, t12, t13), val);
             if (bot)
-                val = abc_blend(wt[ntz - LC - iz - 1],
-                                abc_extrapolate(p1c, zn[H - 1], zn[H - 2], c.p0g, ip, -u.pitch, t11, t12, t13), val);
+                val = abc_blend(bload(rw, 0u, (ntz - LC - iz - 1) * 4),
+                                abc_extrapolate(p1c, zn[H - 1], zn[H - 2], rp0, vo, ro, -pstep, t11, t12, t13), val);
             if (lef)
-                val = abc_blend(wt[ix - LC], abc_extrapolate(p1c, sp[1], sp[2], c.p0g, ip, 1, t11, t12, t13), val);
+                val = abc_blend(bload(rw, (unsigned)(ix - LC) * 4u, 0), abc_extrapolate(p1c, sp[1], sp[2], rp0, vo, ro, 4, t11, t12, t13), val);
             if (rig)
-                val = abc_blend(wt[ntx - LC - ix - 1], abc_extrapolate(p1c, sp[-1], sp[-2], c.p0g, ip, -1, t11, t12, t13),
+                val = abc_blend(bload(rw, (unsigned)(ntx - LC - ix - 1) * 4u, 0), abc_extrapolate(p1c, sp[-1], sp[-2], rp0, vo, ro, -4, t11, t12, t13),
                                 val);
         }
     }
-    const float dt = u.dt;
+    const float dt = kc.dt;
     if (c.mode != kAdj) {
         // S3: source after the ABC.
         if (src_row && ix == u.s_ix) val = val + G(u.wav)[c.it] * dt * dt * v * v;
@@ -228,7 +245,7 @@ __device__ __forceinline__ float point_value(const UnitDesc& u, const StepCtx& c
         const int d = ix - u.r_ix0;
         if (d >= 0 && d % u.dr == 0 && d / u.dr < u.nr) {
             const int ri = c.it * u.nr + d / u.dr;
-            const float res = u.run_fwi ? G(u.syn)[ri] - G(u.obs)[ri] : G(u.obs)[ri];
+            const float res = kc.run_fwi ? G(u.syn)[ri] - G(u.obs)[ri] : G(u.obs)[ri];
             val = val + res * dt * dt * v * v;
         }
     }
@@ -239,11 +256,12 @@ __device__ __forceinline__ float point_value(const UnitDesc& u, const StepCtx& c
 // (forward-like steps) or imaging condition (adjoint steps), and the field.
 // vo/ro: lane/row byte offsets in padded rows; go: byte offset of the row in
 // inner-grid arrays; inner: the point lies in the inner grid.
-__device__ __forceinline__ void store_point(const UnitDesc& u, const StepCtx& c, float val, float p0c, float p1c,
+__device__ __forceinline__ void store_point(const UnitDesc& u, const PointConsts& kc, const StepCtx& c, float val,
+                                            float p0c, float p1c,
                                             float v, int ix, unsigned vo, int ro, int go, Rsrc grad, bool rec_row,
                                             bool inner)
 {
-    const float dt = u.dt;
+    const float dt = kc.dt;
     const unsigned gvo = (unsigned)(ix - u.pml) * 4u;
     if (c.mode != kAdj) {
         if (c.has_ck) {
@@ -252,7 +270,7 @@ __device__ __forceinline__ void store_point(const UnitDesc& u, const StepCtx& c,
         }
         if (c.has_save && inner) {
             float sv = val;
-            if (u.run_fwi) sv = div2_const((val - 2.f * p1c) + p0c, dt, u.rdt, div_hi(u));
+            if (kc.run_fwi) sv = div2_const((val - 2.f * p1c) + p0c, dt, kc.rdt, kc.hi);
             bstore(sv, c.save, gvo, go);
         }
         if (c.mode == kFwd && rec_row) {
@@ -263,13 +281,13 @@ __device__ __forceinline__ void store_point(const UnitDesc& u, const StepCtx& c,
         const float sv = bload(c.save, gvo, go);
         const float gprev = c.first_image ? 0.f : bload(grad, gvo, go);
         float gnew;
-        if (u.run_fwi) {
+        if (kc.run_fwi) {
             const double vd = (double)v;
             const float v3 = (float)(vd * vd * vd);
-            const double term = -2.0 / (double)v3 * (double)sv * (double)val * (double)u.k * (double)dt;
+            const double term = -2.0 / (double)v3 * (double)sv * (double)val * (double)kc.k * (double)dt;
             gnew = (float)((double)gprev + term);
         } else {
-            gnew = gprev + sv * val * (float)u.k * dt;
+            gnew = gprev + sv * val * (float)kc.k * dt;
         }
         bstore(gnew, grad, gvo, go);
     }
@@ -313,10 +331,10 @@ __device__ __forceinline__ void process_item(const UnitDesc& u, int s, int item,
         pre1 = pre0 + fe;
     }
     const FieldPtrs f = rotate_ptrs(base, fe, pre0, pre1, si.q);
-    const StepCtx cA = make_step(u, si.mode, si.it, si.seg, f.p0, f.out, margin);
+    const StepCtx cA = make_step(u, si.mode, si.it, si.seg, f.out, margin);
     const int itB = si.mode == kAdj ? si.it - 1 : si.it + 1;
     StepCtx cB;
-    if (pair) cB = make_step(u, si.mode, itB, si.seg, f.p1, base + fe * ((si.q + 1) & 3), margin);
+    if (pair) cB = make_step(u, si.mode, itB, si.seg, base + fe * ((si.q + 1) & 3), margin);
 
     const int seg = item / u.tiles_x;
     const int x0 = (item - seg * u.tiles_x) * u.stripw;
@@ -338,6 +356,8 @@ __device__ __forceinline__ void process_item(const UnitDesc& u, int s, int item,
     // Every lane of the wave inner and away from the bands (for both steps).
     const bool strip_inside = (x0 - loff) >= pml && (x0 - loff + 64) <= ntx - pml;
     const bool inner_x = ix >= pml && ix < ntx - pml;
+    const PointConsts kc = point_consts(u);
+    const int r_iz = u.r_iz, s_iz = u.s_iz, nx4 = u.nx * 4;
 
     const Rsrc rp1 = make_rsrc(f.p1 - margin);
     const Rsrc rp0 = make_rsrc(f.p0 - margin);
@@ -402,16 +422,18 @@ __device__ __forceinline__ void process_item(const UnitDesc& u, int s, int item,
             float valA = 0.f;
             if (za >= 0 && za < ntz) {
                 const bool row_inner = za >= pml && za < ntz - pml;
-                const bool rec_row = za == u.r_iz;
-                const bool src_row = za == u.s_iz;
+                const bool rec_row = za == r_iz;
+                const bool src_row = za == s_iz;
                 const float* sp = row1 + li;
                 if (strip_inside && row_inner) {
-                    valA = point_value<LC, true>(u, cA, ix, za, zA, sp, p0r[r], vh[H + r], src_row, rec_row);
+                    valA = point_value<LC, true>(u, kc, cA, ix, za, zA, sp, p0r[r], vh[H + r], src_row, rec_row, rp0, vo,
+                                                  row_off(za));
                 } else if (lane_ok) {
-                    valA = point_value<LC, false>(u, cA, ix, za, zA, sp, p0r[r], vh[H + r], src_row, rec_row);
+                    valA = point_value<LC, false>(u, kc, cA, ix, za, zA, sp, p0r[r], vh[H + r], src_row, rec_row, rp0, vo,
+                                                   row_off(za));
                 }
                 if (owned && za >= zb && za < ze)
-                    store_point(u, cA, valA, p0r[r], zA[H], vh[H + r], ix, vo, row_off(za), (za - pml) * u.nx * 4,
+                    store_point(u, kc, cA, valA, p0r[r], zA[H], vh[H + r], ix, vo, row_off(za), (za - pml) * nx4,
                                 rg, rec_row, row_inner && inner_x);
             }
             w2[2 * H + r] = valA;
@@ -425,19 +447,20 @@ __device__ __forceinline__ void process_item(const UnitDesc& u, int s, int item,
                 row2[li] = zB[H];
                 __builtin_amdgcn_wave_barrier();
                 const bool row_inner = zbr >= pml && zbr < ntz - pml;
-                const bool rec_row = zbr == u.r_iz;
-                const bool src_row = zbr == u.s_iz;
+                const bool rec_row = zbr == r_iz;
+                const bool src_row = zbr == s_iz;
                 const float* sp = row2 + li;
                 const float p0b = w1[r];   // step A's p1 at row zbr
                 const float vb = vh[r];
                 float valB = 0.f;
                 if (strip_inside && row_inner) {
-                    valB = point_value<LC, true>(u, cB, ix, zbr, zB, sp, p0b, vb, src_row, rec_row);
+                    valB = point_value<LC, true>(u, kc, cB, ix, zbr, zB, sp, p0b, vb, src_row, rec_row, rp1, vo, row_off(zbr));
                 } else if (lane_ok) {
-                    valB = point_value<LC, false>(u, cB, ix, zbr, zB, sp, p0b, vb, src_row, rec_row);
+                    valB = point_value<LC, false>(u, kc, cB, ix, zbr, zB, sp, p0b, vb, src_row, rec_row, rp1, vo,
+                                                   row_off(zbr));
                 }
                 if (owned)
-                    store_point(u, cB, valB, p0b, zB[H], vb, ix, vo, row_off(zbr), (zbr - pml) * u.nx * 4, rg,
+                    store_point(u, kc, cB, valB, p0b, zB[H], vb, ix, vo, row_off(zbr), (zbr - pml) * nx4, rg,
                                 rec_row, row_inner && inner_x);
                 __builtin_amdgcn_wave_barrier();
             }
