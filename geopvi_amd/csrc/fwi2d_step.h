@@ -58,12 +58,13 @@ __device__ __forceinline__ float div_hi(const UnitDesc& u) { return u.fastdiv ? 
 // stores alias the descriptor.)
 struct PointConsts {
     float dx, dz, dt, rdx, rdz, rdt, hi;
-    int run_fwi, k;
+    int run_fwi, k, ntx, ntz, pml, pitch4;
 };
 
 __device__ __forceinline__ PointConsts point_consts(const UnitDesc& u)
 {
-    return PointConsts{u.dx, u.dz, u.dt, u.rdx, u.rdz, u.rdt, div_hi(u), u.run_fwi, u.k};
+    return PointConsts{u.dx, u.dz, u.dt, u.rdx, u.rdz, u.rdt, div_hi(u), u.run_fwi, u.k, u.ntx, u.ntz, u.pml,
+                       u.pitch * 4};
 }
 
 __device__ __forceinline__ float cdiv_fast(float a, float b, float y)
@@ -196,7 +197,7 @@ __device__ __forceinline__ float point_value(const UnitDesc& u, const PointConst
     constexpr int H = LC - 1;
     const float hi = kc.hi;
     const float p1c = zn[H];
-    const int ntx = u.ntx, ntz = u.ntz, pml = u.pml;
+    const int ntx = kc.ntx, ntz = kc.ntz, pml = kc.pml;
     float val = p1c;
     if (INSIDE || (iz >= LC && iz < ntz - LC && ix >= LC && ix < ntx - LC)) {
         // S1: (2*p1 - p0) + (v*dt)^2 * (d2x/dx/dx + d2z/dz/dz)
@@ -217,7 +218,7 @@ __device__ __forceinline__ float point_value(const UnitDesc& u, const PointConst
         const bool lef = ix >= LC && ix < pml;
         const bool rig = ix >= ntx - pml && ix < ntx - LC;
         if (top || bot || lef || rig) {
-            const int pstep = u.pitch * 4;
+            const int pstep = kc.pitch4;
             const float sc = div1_const(v * kc.dt, kc.dx, kc.rdx, hi);
             const float t11 = ((2.f - sc) * (1.f - sc)) / 2.f;
             const float t12 = sc * (2.f - sc);
@@ -262,7 +263,7 @@ __device__ __forceinline__ void store_point(const UnitDesc& u, const PointConsts
                                             bool inner)
 {
     const float dt = kc.dt;
-    const unsigned gvo = (unsigned)(ix - u.pml) * 4u;
+    const unsigned gvo = (unsigned)(ix - kc.pml) * 4u;
     if (c.mode != kAdj) {
         if (c.has_ck) {
             bstore(p0c, c.ck0, vo, ro);
