@@ -74,27 +74,38 @@ __device__ __forceinline__ float cdiv_fast(float a, float b, float y)
     return __builtin_fmaf(r, y, q0);
 }
 
+// The fallback is taken by a wave-wide vote: when every active lane is in
+// range (the common case) the quotient costs one scalar branch; otherwise
+// the whole wave forms the IEEE quotient and each lane picks its own.
+__device__ __forceinline__ bool any_lane(bool x) { return __builtin_amdgcn_ballot_w64(x) != 0; }
+
 // (a / b) / b, as the reference writes d2p/dx/dx and (.)/dt/dt.
 __device__ __forceinline__ float div2_const(float a, float b, float y, float hi)
 {
     const float q = cdiv_fast(cdiv_fast(a, b, y), b, y);
-    if (__builtin_expect(cdiv_in_range(a, hi), 1)) return q;
-    return (a / b) / b;
+    const bool ok = cdiv_in_range(a, hi);
+    if (__builtin_expect(!any_lane(!ok), 1)) return q;
+    const float slow = (a / b) / b;
+    return ok ? q : slow;
 }
 
 __device__ __forceinline__ float div1_const(float a, float b, float y, float hi)
 {
     const float q = cdiv_fast(a, b, y);
-    if (__builtin_expect(cdiv_in_range(a, hi), 1)) return q;
-    return a / b;
+    const bool ok = cdiv_in_range(a, hi);
+    if (__builtin_expect(!any_lane(!ok), 1)) return q;
+    const float slow = a / b;
+    return ok ? q : slow;
 }
 
 // (a / bx) / bx + (c / bz) / bz with one range test for both quotients.
 __device__ __forceinline__ float lap_const(float a, float bx, float yx, float c, float bz, float yz, float hi)
 {
     const float q = cdiv_fast(cdiv_fast(a, bx, yx), bx, yx) + cdiv_fast(cdiv_fast(c, bz, yz), bz, yz);
-    if (__builtin_expect(cdiv_in_range(a, hi) & cdiv_in_range(c, hi), 1)) return q;
-    return (a / bx) / bx + (c / bz) / bz;
+    const bool ok = cdiv_in_range(a, hi) & cdiv_in_range(c, hi);
+    if (__builtin_expect(!any_lane(!ok), 1)) return q;
+    const float slow = (a / bx) / bx + (c / bz) / bz;
+    return ok ? q : slow;
 }
 
 // Raw buffer access: a 128-bit scalar resource (base address, byte range,
