@@ -30,3 +30,17 @@ def case(name: str, seed: int = 0, smooth_sigma: float = None, **overrides):
 # <= 1e-5, gradient relative L2 <= 1e-4.
 MISFIT_RTOL = 1e-5
 GRAD_RTOL = 1e-4
+
+
+def fwi_config(p):
+    """configparser object with the [FWI] section the reference's Posterior
+    reads (examples/fwi2d/input/config.ini keys) for run parameters p."""
+    import configparser
+    cfg = configparser.ConfigParser()
+    keys = [("nx", "nx"), ("nz", "nz"), ("pml0", "pml"), ("Lc", "Lc"), ("laplace_slover", "laplace_slover"),
+            ("ns", "ns"), ("nt", "nt"), ("ds", "ds"), ("ns0", "ns0"), ("depths", "depths"), ("depthr", "depthr"),
+            ("nr", "nr"), ("dr", "dr"), ("nr0", "nr0"), ("nt_interval", "nt_interval"), ("dx", "dx"),
+            ("dz", "dz"), ("dt", "dt"), ("f0", "f0")]
+    cfg["FWI"] = {key: (repr(getattr(p, name)) if name in ("dx", "dz", "dt", "f0") else str(getattr(p, name)))
+                  for name, key in keys}
+    return cfg

@@ -1,0 +1,94 @@
+"""End-to-end variational inversion on the device vs a CPU run.
+
+The GPU run keeps the flows, samples and gradients on the device and takes
+misfits/gradients from the batched HIP solver through Posterior.log_prob's
+device path.  The CPU run uses the same flows on the host and the C oracle
+as the likelihood (the reference's float32 misfit, posterior.py:101-110).
+Both draw the same base samples (one seeded CPU generator), so after a few
+Adam iterations the variational parameters and the per-iteration -ELBO must
+agree to the solver's parity (misfit 1e-5, gradient 1e-4 -> tiny parameter
+differences after Adam's normalised steps).
+"""
+import copy
+
+import numpy as np
+import pytest
+import torch
+
+from helpers import fwi_config
+from geopvi_amd import synth
+from geopvi_amd.forward.fwi2d.posterior import Posterior
+from geopvi_amd.nfvi import FlowsBasedDistribution, Linear, Real2Constr, VariationalInversion, psvi_mask_2D
+from geopvi_amd.prior import Uniform
+
+pytestmark = pytest.mark.gpu
+
+
+class OracleLikelihood(torch.autograd.Function):
+    """CPU log-likelihood term (-misfit) with the C oracle's misfit/gradient."""
+
+    @staticmethod
+    def forward(ctx, z, oracle_mod, p, obs):
+        zs = z.detach().numpy()
+        loss = np.zeros(zs.shape[0])
+        grad = np.zeros_like(zs)
+        for i, v in enumerate(zs):
+            syn, g = oracle_mod.fwi(p, v.astype(np.float32), obs)
+            loss[i] = oracle_mod.misfit_f32(syn, obs)
+            grad[i] = -g
+        ctx.save_for_backward(torch.from_numpy(grad))
+        return torch.from_numpy(loss)
+
+    @staticmethod
+    def backward(ctx, go):
+        (grad,) = ctx.saved_tensors
+        return go[:, None] * grad, None, None, None
+
+
+@pytest.fixture()
+def float64_default():
+    old = torch.get_default_dtype()
+    torch.set_default_dtype(torch.float64)
+    yield
+    torch.set_default_dtype(old)
+
+
+@pytest.mark.parametrize("kernel", ["diagonal", "structured"])
+def test_inversion_device_matches_cpu_oracle_run(oracle_mod, tmp_path, float64_default, kernel):
+    p = synth.workload_params("tiny", nt=200)
+    vt = synth.gradient_anomaly_model(p.nz, p.nx, seed=4)
+    vs = synth.smooth(vt, 2.0)
+    obs = oracle_mod.forward(p, vt)
+    sigma = 0.05 * float(np.sqrt(np.mean(obs.astype(np.float64) ** 2)))
+    dim = p.ni
+    # bounds around the model's range (1200..3800 m/s), inside the stencil's
+    # stability limit v*dt/dx <= 0.575
+    lo, up = np.full(dim, 1000.0), np.full(dim, 5000.0)
+    mask = psvi_mask_2D(np.ones((3, 3)), dim, nx=p.nx, nz=p.nz)
+    u0 = np.log(vs.reshape(-1) - lo) - np.log(up - vs.reshape(-1))      # logit of the starting model
+    assert np.isfinite(u0).all()
+    param = np.concatenate([u0, np.full(dim, 0.02)] + ([np.zeros(mask.size)] if kernel == "structured" else []))
+    flows = [Linear(dim, kernel=kernel, mask=mask, param=param), Real2Constr(dim, lower=lo, upper=up)]
+    prior = Uniform(lower=lo, upper=up)
+    n_iter, nsample, lr = 3, 4, 0.01
+
+    # CPU run with the oracle likelihood
+    q_cpu = FlowsBasedDistribution(copy.deepcopy(flows), generator=torch.Generator().manual_seed(11))
+    logp_cpu = lambda z: -OracleLikelihood.apply(z, oracle_mod, p, obs) / sigma ** 2 + prior.log_prob(z)  # noqa: E731
+    loss_cpu = VariationalInversion(q_cpu, logp_cpu).update(lr=lr, n_iter=n_iter, nsample=nsample)
+
+    # device run
+    post = Posterior(obs, fwi_config(p), vel_fixed=1500.0, sigma=sigma, log_prior=prior.log_prob,
+                     paramfile=str(tmp_path / "input_params.txt"))
+    q_dev = FlowsBasedDistribution(copy.deepcopy(flows), generator=torch.Generator().manual_seed(11)).cuda()
+    loss_dev = VariationalInversion(q_dev, post.log_prob).update(lr=lr, n_iter=n_iter, nsample=nsample)
+
+    print("loss cpu", loss_cpu, "dev", loss_dev, flush=True)
+    np.testing.assert_allclose(loss_dev, loss_cpu, rtol=1e-5)
+    for (name, a), (_, b) in zip(q_dev.named_parameters(), q_cpu.named_parameters()):
+        a, b = a.detach().cpu().numpy(), b.detach().numpy()
+        diff = np.abs(a - b).max()
+        print(name, "max |dev - cpu|", diff, flush=True)
+        assert diff <= lr * 1e-3, name
+    # the parameters moved (the test is not vacuous)
+    assert np.abs(q_cpu.flows[0].u.detach().numpy() - u0).max() > 0.5 * lr
