@@ -57,6 +57,8 @@ def parse():
     ap.add_argument("--serve-low-rate", type=float, default=10.0, help="sub-saturation run rate (0 = skip)")
     ap.add_argument("--serve-low-requests", type=int, default=60)
     ap.add_argument("--cpu-shots", type=int, default=20)
+    ap.add_argument("--no-vi", action="store_true", help="skip the end-to-end inversion leg")
+    ap.add_argument("--vi-iterations", type=int, default=2)
     ap.add_argument("--seed", type=int, default=1)
     return ap.parse_args()
 
@@ -116,6 +118,69 @@ def serve_stream(args, rate, nreq, seed, dev):
             "mean_ms": round(float(lat.mean()) * 1e3, 1), "makespan_s": round(span, 3),
             "samples_per_s": round(total_samples / span, 3),
             "achieved_GBs": round(total_bytes / span / 1e9, 1)}
+
+
+def vi_leg(args, dev):
+    """End-to-end variational inversion iterations on the c1 problem
+    (200x400 grid, 80 000 parameters, 20 shots, 2000 steps) with a
+    Real2Constr bound of [1400, 4800] m/s: mean-field ADVI (diagonal Linear)
+    and PSVI (Linear with the 12 bands of a 5x5 correlation template), 32
+    samples per iteration, everything on the device; checkpointed adjoint
+    (interval minimising memory).  One untimed warm-up iteration, then
+    `vi_iterations` timed ones per method."""
+    import torch
+    from geopvi_amd import synth
+    from geopvi_amd.engine import best_ckpt_interval
+    from geopvi_amd.forward.fwi2d import solver
+    from geopvi_amd.forward.fwi2d.posterior import Posterior
+    from geopvi_amd.nfvi import FlowsBasedDistribution, Linear, Real2Constr, VariationalInversion, psvi_mask_2D
+    from geopvi_amd.prior import Uniform
+    import configparser
+
+    old = torch.get_default_dtype()
+    torch.set_default_dtype(torch.float64)
+    try:
+        p = synth.workload_params("c1")
+        vt = synth.true_model("c1", seed=args.seed)
+        obs = solver.forward_batch(p, torch.from_numpy(vt.astype(np.float32).reshape(1, -1)).to(dev))[0]
+        obs_np = obs.cpu().numpy()
+        sigma = 0.01 * float(np.sqrt(np.mean(obs_np.astype(np.float64) ** 2)))
+        cfg = configparser.ConfigParser()
+        names = [("nx", "nx"), ("nz", "nz"), ("pml0", "pml"), ("Lc", "Lc"), ("laplace_slover", "laplace_slover"),
+                 ("ns", "ns"), ("nt", "nt"), ("ds", "ds"), ("ns0", "ns0"), ("depths", "depths"),
+                 ("depthr", "depthr"), ("nr", "nr"), ("dr", "dr"), ("nr0", "nr0"), ("nt_interval", "nt_interval"),
+                 ("dx", "dx"), ("dz", "dz"), ("dt", "dt"), ("f0", "f0")]
+        cfg["FWI"] = {k: (repr(getattr(p, n)) if n in ("dx", "dz", "dt", "f0") else str(getattr(p, n)))
+                      for n, k in names}
+        dim, lo, up = p.ni, 1400.0, 4800.0
+        vs = np.clip(synth.smooth(vt, 8.0).reshape(-1), lo + 50, up - 50)
+        u0 = np.log(vs - lo) - np.log(up - vs)
+        ckpt = best_ckpt_interval(p)
+        prior = Uniform(lower=np.full(dim, lo), upper=np.full(dim, up))
+        post = Posterior(obs_np, cfg, sigma=sigma, log_prior=prior.log_prob,
+                         paramfile=os.path.join("/tmp", f"bench_vi_params_{os.getpid()}.txt"), ckpt_interval=ckpt)
+        out = {"problem": "200x400 (80000 parameters), 20 shots, 2000 steps, bounds [1400, 4800]",
+               "nsample": 32, "ckpt_interval": ckpt}
+        for kernel in ("diagonal", "structured"):
+            mask = psvi_mask_2D(np.ones((5, 5)), dim, nx=p.nx, nz=p.nz)
+            extra = [np.zeros(mask.size)] if kernel == "structured" else []
+            flows = [Linear(dim, kernel=kernel, mask=mask, param=np.concatenate([u0, np.full(dim, 0.01)] + extra)),
+                     Real2Constr(dim, lower=lo, upper=up)]
+            q = FlowsBasedDistribution(flows, generator=torch.Generator(device=dev).manual_seed(args.seed)).to(dev)
+            vi = VariationalInversion(q, post.log_prob)
+            vi.update(lr=0.002, n_iter=1, nsample=32)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            loss = vi.update(lr=0.002, n_iter=args.vi_iterations, nsample=32)
+            torch.cuda.synchronize()
+            dt_it = (time.perf_counter() - t0) / args.vi_iterations
+            name = "advi" if kernel == "diagonal" else "psvi"
+            out[name] = {"s_per_iteration": round(dt_it, 3), "samples_per_s": round(32 / dt_it, 2),
+                         "loss": [round(v, 1) for v in loss]}
+            log(f"vi {name}: {dt_it:.3f} s/iteration")
+        return out
+    finally:
+        torch.set_default_dtype(old)
 
 
 def main():
@@ -266,6 +331,10 @@ def main():
             serve["sub_saturation"] = serve_stream(args, args.serve_low_rate, args.serve_low_requests,
                                                    rng_seed + 400, dev)
         out["serving"] = serve
+
+    if not args.no_vi:
+        out["vi"] = vi_leg(args, dev)
+        torch.cuda.empty_cache()
 
     if not args.no_cpu:
         import oracle
