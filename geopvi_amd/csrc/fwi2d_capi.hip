@@ -156,7 +156,7 @@ bool fastdiv_ok(float b, float* recip)
 // the buffer stride and `origin` the offset of row 0 inside a buffer.
 struct Geom {
     int lc, pml, ntx, ntz, pitch, nx, nz, ni, nt, k, nr, ns, nsave, pml0;
-    int variant, tile_x, tile_z, tiles_x, tiles_z, tpu, zseg, tb, stripw;
+    int variant, tile_x, tile_z, tiles_x, tiles_z, tpu, zseg, tb, stripw, band_w, band_tz;
     int64_t fe, origin;
     float fd[8];
     float rdx, rdz, rdt;
@@ -185,7 +185,36 @@ void make_geom(const gpvi_fwi2d_params* p, int variant, int tb, Geom* g)
     g->tile_x = g->stripw;
     g->tiles_x = (g->ntx + g->tile_x - 1) / g->tile_x;
     g->tiles_z = (g->ntz + g->tile_z - 1) / g->tile_z;
-    g->tpu = g->tiles_x * g->tiles_z;
+    // One step per launch: the absorbing side bands (columns < pml and
+    // >= ntx - pml) go to narrow side-band items, w columns wide (16 or 32,
+    // >= pml; 64/w rows per wave step), so most lanes of a side-band wave
+    // are band points, and the row strips over [w_left, ntx - w_right) are
+    // all away from the side bands.  The two widths are chosen so the row
+    // strips tile their range with the fewest idle lanes; band_w packs them
+    // as w_left | w_right << 8.
+    g->band_w = 0;
+    g->band_tz = 0;
+    if (g->tb == 1 && g->pml <= 32) {
+        int best = -1, best_waste = 1 << 30;
+        for (int wl = 16; wl <= 32; wl += 16)
+            for (int wr = 16; wr <= 32; wr += 16) {
+                if (wl < g->pml || wr < g->pml || g->ntx < wl + wr + kStripWidth) continue;
+                const int inner = g->ntx - wl - wr;
+                const int n = (inner + kStripWidth - 1) / kStripWidth;
+                const int waste = n * kStripWidth - inner + (wl - 16) + (wr - 16);
+                if (waste < best_waste) {
+                    best_waste = waste;
+                    best = wl | (wr << 8);
+                }
+            }
+        if (best >= 0) {
+            const int wl = best & 0xff, wr = best >> 8;
+            g->band_w = best;
+            g->band_tz = g->tiles_z;
+            g->tiles_x = (g->ntx - wl - wr + kStripWidth - 1) / kStripWidth;
+        }
+    }
+    g->tpu = 2 * g->band_tz + g->tiles_x * g->tiles_z;
     g->pml0 = p->pml0;
     fd_coeffs(g->lc, g->fd);
     const bool okx = fastdiv_ok(p->dx, &g->rdx);
@@ -314,6 +343,8 @@ JobDesc make_job(const gpvi_fwi2d_params* p, const Geom& g, const Sched& sc, int
     }
     j.tb = g.tb;
     j.stripw = g.stripw;
+    j.band_w = g.band_w;
+    j.band_tz = g.band_tz;
     j.run_fwi = run_fwi ? 1 : 0;
     j.store = store;
     j.dt = p->dt; j.dx = p->dx; j.dz = p->dz;
@@ -565,6 +596,7 @@ static UnitDesc plain_unit(const gpvi_fwi2d_params* p, const Geom& g)
     u.r_ix0 = g.pml + p->nr0; u.r_iz = g.pml + p->depthr; u.s_iz = g.pml + p->depths;
     u.nseg = 1; u.ckpt = g.nt; u.last_len = g.nt; u.nslot = g.nsave;
     u.tiles_x = g.tiles_x; u.tiles_z = g.tiles_z; u.zseg = g.zseg;
+    u.band_w = g.band_w; u.band_tz = g.band_tz;
     u.dt = p->dt; u.dx = p->dx; u.dz = p->dz;
     u.rdx = g.rdx; u.rdz = g.rdz; u.rdt = g.rdt; u.fastdiv = g.fastdiv;
     for (int c = 0; c < 8; ++c) u.fd[c] = g.fd[c];

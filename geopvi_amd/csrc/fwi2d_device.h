@@ -65,13 +65,15 @@ struct UnitDesc {
     int nseg, ckpt, last_len, nslot;   // segments of the adjoint schedule
     int tiles_x, tiles_z, start, s_off;  // tiles_x: column strips, tiles_z: row segments;
                                          // local macro step s = global - start + s_off
-    int zseg, tb, stripw, pad4;          // rows per strip segment (one wavefront's work item),
-                                         // time steps per macro (1 or 2), columns per strip
+    int zseg, tb, stripw, band_w;        // rows per strip segment (one wavefront's work item),
+                                         // time steps per macro (1 or 2), columns per strip,
+                                         // width of the side-band items (0: none)
     int total, run_fwi, grad_init, store;  // total macro steps; grad_init: first imaging step writes;
                                            // store: keep wavefields
     float dt, dx, dz, rdt;                 // rdx/rdz/rdt = RN(1/dx) ...: exact constant division
     float rdx, rdz;
-    int fastdiv, pad1;                     // 1: the host verified the reciprocal division for dx, dz, dt
+    int fastdiv, band_tz;                  // 1: the host verified the reciprocal division for dx, dz, dt;
+                                           // side-band items per side
     float fd[8];
     int64_t field_elems;  // stride between field buffers: (ntz + 2*kPadRows) * pitch
     const float* wtab;    // ABC taper w[i] = 1 - i/pml0, i < pml0

@@ -22,10 +22,10 @@ struct JobDesc {
     int s_ix0, ds, s_iz, nseg;
     int ckpt, last_len, nslot, start;
     int s_off, total, run_fwi, store;
-    int zseg, tb, stripw, pad4;
+    int zseg, tb, stripw, band_w;
     float dt, dx, dz, rdt;
     float rdx, rdz;
-    int fastdiv, pad1;
+    int fastdiv, band_tz;
     float fd[8];
     int64_t fe;
     const float* wtab;

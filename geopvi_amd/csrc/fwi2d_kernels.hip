@@ -14,12 +14,12 @@ template <int LC, int NTH>
 __global__ __launch_bounds__(NTH) void step_single_kernel(UnitDesc u, int s)
 {
     constexpr int NW = NTH / 64;
-    __shared__ float rows[NW][kStripWidth + 2 * (LC - 1)];
+    __shared__ float rows[NW][wave_lds_floats(LC, 1)];
     // readfirstlane: the compiler then knows the wave index is uniform, so
     // everything derived from it lives in scalar registers.
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int item = blockIdx.x * NW + wave;
-    if (item < u.tiles_x * u.tiles_z) process_item<LC, 1>(u, s, item, rows[wave]);
+    if (item < 2 * u.band_tz + u.tiles_x * u.tiles_z) process_item<LC, 1>(u, s, item, rows[wave]);
 }
 
 // Many units per launch, TB time steps per launch.  A grid of independent
@@ -31,12 +31,12 @@ __global__ __launch_bounds__(NTH) void step_single_kernel(UnitDesc u, int s)
 // item count come from device memory so a captured graph segment can be
 // replayed without re-instantiation.
 template <int LC, int NTH, int TB>
-__global__ __launch_bounds__(NTH) void step_multi_kernel(const UnitDesc* __restrict__ units,
+__global__ __launch_bounds__(NTH) __attribute__((amdgpu_waves_per_eu(TB == 1 && LC <= 5 ? 6 : 1, 8))) void step_multi_kernel(const UnitDesc* __restrict__ units,
                                                          const TileRef* __restrict__ tiles,
                                                          const StepCtl* __restrict__ ctl, int step_off)
 {
     constexpr int NW = NTH / 64;
-    __shared__ float rows[NW][TB * (kStripWidth + 2 * (LC - 1))];
+    __shared__ float rows[NW][wave_lds_floats(LC, TB)];
     // readfirstlane: the wave index, the work item and the unit descriptor
     // address are then known to be uniform (scalar loads, scalar registers).
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -127,7 +127,7 @@ __global__ void tables_kernel(float* __restrict__ wav, int nt, float dt, float f
 __global__ void build_units_kernel(JobDesc j, UnitDesc* __restrict__ units, TileRef* __restrict__ tiles)
 {
     const int nunit = j.nsample * j.ns;
-    const int tpu = j.tiles_x * j.tiles_z;
+    const int tpu = 2 * j.band_tz + j.tiles_x * j.tiles_z;
     const int64_t ntile = (int64_t)nunit * tpu;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < ntile;
          i += (int64_t)gridDim.x * blockDim.x) {
@@ -144,10 +144,10 @@ __global__ void build_units_kernel(JobDesc j, UnitDesc* __restrict__ units, Tile
     u.r_ix0 = j.r_ix0; u.r_iz = j.r_iz; u.s_ix = j.s_ix0 + sh * j.ds; u.s_iz = j.s_iz;
     u.nseg = j.nseg; u.ckpt = j.ckpt; u.last_len = j.last_len; u.nslot = j.nslot;
     u.tiles_x = j.tiles_x; u.tiles_z = j.tiles_z; u.start = j.start; u.s_off = j.s_off;
-    u.zseg = j.zseg; u.tb = j.tb; u.stripw = j.stripw; u.pad4 = 0;
+    u.zseg = j.zseg; u.tb = j.tb; u.stripw = j.stripw; u.band_w = j.band_w;
     u.total = j.total; u.run_fwi = j.run_fwi; u.grad_init = 1; u.store = j.store;
     u.dt = j.dt; u.dx = j.dx; u.dz = j.dz; u.rdt = j.rdt;
-    u.rdx = j.rdx; u.rdz = j.rdz; u.fastdiv = j.fastdiv; u.pad1 = 0;
+    u.rdx = j.rdx; u.rdz = j.rdz; u.fastdiv = j.fastdiv; u.band_tz = j.band_tz;
     for (int c = 0; c < 8; ++c) u.fd[c] = j.fd[c];
     u.field_elems = j.fe;
     u.wtab = j.wtab;

@@ -99,10 +99,12 @@ __device__ __forceinline__ float div1_const(float a, float b, float y, float hi)
 }
 
 // (a / bx) / bx + (c / bz) / bz with one range test for both quotients.
-__device__ __forceinline__ float lap_const(float a, float bx, float yx, float c, float bz, float yz, float hi)
+// Lanes that are not `active` (their result is discarded) do not vote.
+__device__ __forceinline__ float lap_const(float a, float bx, float yx, float c, float bz, float yz, float hi,
+                                           bool active = true)
 {
     const float q = cdiv_fast(cdiv_fast(a, bx, yx), bx, yx) + cdiv_fast(cdiv_fast(c, bz, yz), bz, yz);
-    const bool ok = cdiv_in_range(a, hi) & cdiv_in_range(c, hi);
+    const bool ok = (cdiv_in_range(a, hi) & cdiv_in_range(c, hi)) | !active;
     if (__builtin_expect(!any_lane(!ok), 1)) return q;
     const float slow = (a / bx) / bx + (c / bz) / bz;
     return ok ? q : slow;
@@ -203,7 +205,8 @@ __device__ __forceinline__ StepCtx make_step(const UnitDesc& u, int mode, int it
 template <int LC, bool INSIDE>
 __device__ __forceinline__ float point_value(const UnitDesc& u, const PointConsts& kc, const StepCtx& c, int ix, int iz,
                                              const float (&zn)[2 * LC - 1], const float* sp, float p0c, float v,
-                                             bool src_row, bool rec_row, Rsrc rp0, unsigned vo, int ro)
+                                             bool src_row, bool rec_row, Rsrc rp0, unsigned vo, int ro,
+                                             bool active = true)
 {
     constexpr int H = LC - 1;
     const float hi = kc.hi;
@@ -219,7 +222,7 @@ __device__ __forceinline__ float point_value(const UnitDesc& u, const PointConst
 #pragma unroll
         for (int m = 1; m < LC; ++m) dzz = dzz + fd_weight(LC, m) * (zn[H + m] + zn[H - m]);
         const float vdt = v * kc.dt;
-        const float lap = lap_const(dxx, kc.dx, kc.rdx, dzz, kc.dz, kc.rdz, hi);
+        const float lap = lap_const(dxx, kc.dx, kc.rdx, dzz, kc.dz, kc.rdz, hi, active);
         val = (2.f * p1c - p0c) + (vdt * vdt) * lap;
     }
     if (!INSIDE) {
@@ -266,15 +269,14 @@ __device__ __forceinline__ float point_value(const UnitDesc& u, const PointConst
 
 // Outputs of one owned point: checkpoint, wavefield store and record
 // (forward-like steps) or imaging condition (adjoint steps), and the field.
-// vo/ro: lane/row byte offsets in padded rows; go: byte offset of the row in
-// inner-grid arrays; inner: the point lies in the inner grid.
+// vo/ro: lane/row byte offsets in padded field rows; gvo/go: lane/row byte
+// offsets in the inner-grid arrays (used when `inner`: the point lies in the
+// inner grid).
 __device__ __forceinline__ void store_point(const UnitDesc& u, const PointConsts& kc, const StepCtx& c, float val,
-                                            float p0c, float p1c,
-                                            float v, int ix, unsigned vo, int ro, int go, Rsrc grad, bool rec_row,
-                                            bool inner)
+                                            float p0c, float p1c, float v, int ix, unsigned vo, int ro,
+                                            unsigned gvo, int go, Rsrc grad, bool rec_row, bool inner)
 {
     const float dt = kc.dt;
-    const unsigned gvo = (unsigned)(ix - kc.pml) * 4u;
     if (c.mode != kAdj) {
         if (c.has_ck) {
             bstore(p0c, c.ck0, vo, ro);
@@ -304,6 +306,101 @@ __device__ __forceinline__ void store_point(const UnitDesc& u, const PointConsts
         bstore(gnew, grad, gvo, go);
     }
     bstore(val, c.out, vo, ro);
+}
+
+// LDS floats one wavefront uses: TB rows of a row strip (64 + 2H each) or
+// the RS row slots of a side-band item (W + 2H each, RS*W = 64, RS <= 4).
+__host__ __device__ constexpr int wave_lds_floats(int lc, int tb)
+{
+    return tb * (64 + 2 * (lc - 1)) > 64 + 8 * (lc - 1) ? tb * (64 + 2 * (lc - 1)) : 64 + 8 * (lc - 1);
+}
+
+// One side-band item (one time step): columns [0, W) (item & 1 == 0) or
+// [ntx - W, ntx) of rows [zb, ze), W = 64/RS >= pml (that side's width in
+// UnitDesc::band_w).
+// Lane l owns column l % W of row zb + RS*g + l / W at wave step g, so every
+// wave step advances RS rows and most lanes are absorbing-band points.  Each
+// lane keeps p1 of its own column around its row in registers (z
+// neighbours); the RS rows of a step are staged in LDS with their halo
+// columns (x neighbours).  Offsets: a lane's row varies across the wave, so
+// every access is a per-lane byte offset from the lane's reference row
+// rb = zb + l/W - H - 4 plus a uniform multiple of the pitch (>= 0).
+template <int LC, int RS>
+__device__ __forceinline__ void process_band(const UnitDesc& u, const PointConsts& kc, const StepCtx& c,
+                                             const FieldPtrs& f, int item, float* lds)
+{
+    constexpr int H = LC - 1;
+    constexpr int W = 64 / RS;
+    constexpr int RWb = W + 2 * H;
+    const int side = item & 1;
+    const int zb = (item >> 1) * u.zseg;
+    const int ze = min(zb + u.zseg, kc.ntz);
+    const int ntx = kc.ntx, ntz = kc.ntz, pml = kc.pml, pitch4 = kc.pitch4;
+    const int lane = threadIdx.x & 63;
+    const int cl = lane % W, j = lane / W;
+    const int col0 = side ? ntx - W : 0;
+    const int ix = col0 + cl;
+    const int64_t margin = (int64_t)kPadRows * u.pitch;
+    const Rsrc rp1 = make_rsrc(f.p1 - margin);
+    const Rsrc rp0 = make_rsrc(f.p0 - margin);
+    const Rsrc rv = make_rsrc(u.vel - margin);
+    const Rsrc rg = make_rsrc(u.grad);
+    const int rb = zb + j - H - 4;
+    const unsigned lb = (unsigned)((rb + kPadRows) * pitch4 + ix * 4);
+    const bool halo_lane = cl < H;
+    const unsigned hslot = side ? cl : W + H + cl;
+    const int hcol = side ? ntx - W - H + cl : W + cl;
+    const unsigned hb = (unsigned)((rb + kPadRows) * pitch4 + hcol * 4);
+    const int ro_abc = 4 * pitch4 + 16;   // inward ABC reads stay at offsets >= 0
+    const int r_iz = u.r_iz, s_iz = u.s_iz, nx = u.nx;
+    const bool inner_x = ix >= pml && ix < ntx - pml;
+    float* myrow = lds + j * RWb;
+    const unsigned li = cl + H;
+
+    // w[m]: p1 at row (lane's row) - H + m; the lane's row at step g is rb + H + 4 + RS*g
+    float w[2 * H + 1];
+#pragma unroll
+    for (int m = 0; m <= 2 * H; ++m) w[m] = bload(rp1, lb, (4 + m) * pitch4);
+    float hal = halo_lane ? bload(rp1, hb, (H + 4) * pitch4) : 0.f;
+    float p0c = bload(rp0, lb, (H + 4) * pitch4);
+    float vc = bload(rv, lb, (H + 4) * pitch4);
+    const int ng = (ze - zb + RS - 1) / RS;
+    for (int g = 0; g < ng; ++g) {
+        const int base = RS * g;
+        const bool more = g + 1 < ng;
+        float nw[RS], nh = 0.f, n0 = 0.f, nv = 0.f;
+        if (more) {
+#pragma unroll
+            for (int k = 0; k < RS; ++k) nw[k] = bload(rp1, lb, (base + 2 * H + 5 + k) * pitch4);
+            nh = halo_lane ? bload(rp1, hb, (base + RS + H + 4) * pitch4) : 0.f;
+            n0 = bload(rp0, lb, (base + RS + H + 4) * pitch4);
+            nv = bload(rv, lb, (base + RS + H + 4) * pitch4);
+        } else {
+#pragma unroll
+            for (int k = 0; k < RS; ++k) nw[k] = 0.f;
+        }
+        const int iz = zb + base + j;
+        myrow[li] = w[H];
+        if (halo_lane) myrow[hslot] = hal;
+        __builtin_amdgcn_wave_barrier();
+        if (iz < ze) {
+            const bool rec_row = iz == r_iz, src_row = iz == s_iz;
+            const unsigned vo = lb + (unsigned)((base + H + 4) * pitch4 - ro_abc);
+            const float val = point_value<LC, false>(u, kc, c, ix, iz, w, myrow + li, p0c, vc, src_row, rec_row, rp0,
+                                                     vo, ro_abc);
+            const bool inner = inner_x && iz >= pml && iz < ntz - pml;
+            const unsigned gv = inner ? (unsigned)(((iz - pml) * nx + (ix - pml)) * 4) : 0u;
+            store_point(u, kc, c, val, p0c, w[H], vc, ix, vo, ro_abc, gv, 0, rg, rec_row, inner);
+        }
+        __builtin_amdgcn_wave_barrier();
+#pragma unroll
+        for (int m = 0; m + RS <= 2 * H; ++m) w[m] = w[m + RS];
+#pragma unroll
+        for (int k = 0; k < RS; ++k) w[2 * H + 1 - RS + k] = nw[k];
+        hal = nh;
+        p0c = n0;
+        vc = nv;
+    }
 }
 
 // One wavefront: one strip segment of one unit at local macro step s.
@@ -348,14 +445,29 @@ __device__ __forceinline__ void process_item(const UnitDesc& u, int s, int item,
     StepCtx cB;
     if (pair) cB = make_step(u, si.mode, itB, si.seg, base + fe * ((si.q + 1) & 3), margin);
 
+    // Side-band items first (UnitDesc::band_w = left width | right width << 8),
+    // then row strips over [left width, ntx - right width).
+    const int bl = TB == 1 ? (u.band_w & 0xff) : 0;
+    const int br = TB == 1 ? (u.band_w >> 8) : 0;
+    if (bl > 0) {
+        if (item < 2 * u.band_tz) {
+            if ((item & 1 ? br : bl) == 16)
+                process_band<LC, 4>(u, point_consts(u), cA, f, item, lds);
+            else
+                process_band<LC, 2>(u, point_consts(u), cA, f, item, lds);
+            return;
+        }
+        item -= 2 * u.band_tz;
+    }
     const int seg = item / u.tiles_x;
-    const int x0 = (item - seg * u.tiles_x) * u.stripw;
+    const int x0 = bl + (item - seg * u.tiles_x) * u.stripw;
     const int zb = seg * u.zseg;
     const int ze = min(zb + u.zseg, ntz);
     const int lane = threadIdx.x & 63;
     const int loff = TB == 2 ? H : 0;             // first owned lane
     const int ix = x0 - loff + lane;              // this lane's column
-    const bool lane_ok = ix >= 0 && ix < ntx;
+    const int x_end = ntx - br;                   // row strips own columns [bl, ntx - br)
+    const bool lane_ok = ix >= 0 && ix < x_end;
     const bool owned = lane_ok && lane >= loff && lane < loff + u.stripw;
     const int cx0 = x0 - loff - H;                // column of LDS slot 0
     const unsigned li = lane + H;                 // this lane's LDS slot
@@ -365,9 +477,11 @@ __device__ __forceinline__ void process_item(const UnitDesc& u, int s, int item,
     const unsigned ho = hslot * 4u;
     const int pitch4 = pitch * 4;
     auto row_off = [&](int z) { return (z + kPadRows) * pitch4 + cx0 * 4; };
-    // Every lane of the wave inner and away from the bands (for both steps).
-    const bool strip_inside = (x0 - loff) >= pml && (x0 - loff + 64) <= ntx - pml;
+    // Every owned lane of the wave inner and away from the bands (for both
+    // steps); lanes past x_end compute values nobody stores.
+    const bool strip_inside = (x0 - loff) >= pml && min(x0 - loff + 64, x_end) <= ntx - pml;
     const bool inner_x = ix >= pml && ix < ntx - pml;
+    const unsigned gvo = (unsigned)(ix - pml) * 4u;   // lane offset in inner-grid arrays (inner_x lanes)
     const PointConsts kc = point_consts(u);
     const int r_iz = u.r_iz, s_iz = u.s_iz, nx4 = u.nx * 4;
 
@@ -439,13 +553,13 @@ __device__ __forceinline__ void process_item(const UnitDesc& u, int s, int item,
                 const float* sp = row1 + li;
                 if (strip_inside && row_inner) {
                     valA = point_value<LC, true>(u, kc, cA, ix, za, zA, sp, p0r[r], vh[H + r], src_row, rec_row, rp0, vo,
-                                                  row_off(za));
+                                                  row_off(za), lane_ok);
                 } else if (lane_ok) {
                     valA = point_value<LC, false>(u, kc, cA, ix, za, zA, sp, p0r[r], vh[H + r], src_row, rec_row, rp0, vo,
                                                    row_off(za));
                 }
                 if (owned && za >= zb && za < ze)
-                    store_point(u, kc, cA, valA, p0r[r], zA[H], vh[H + r], ix, vo, row_off(za), (za - pml) * nx4,
+                    store_point(u, kc, cA, valA, p0r[r], zA[H], vh[H + r], ix, vo, row_off(za), gvo, (za - pml) * nx4,
                                 rg, rec_row, row_inner && inner_x);
             }
             w2[2 * H + r] = valA;
@@ -472,7 +586,7 @@ __device__ __forceinline__ void process_item(const UnitDesc& u, int s, int item,
                                                    row_off(zbr));
                 }
                 if (owned)
-                    store_point(u, kc, cB, valB, p0b, zB[H], vb, ix, vo, row_off(zbr), (zbr - pml) * nx4, rg,
+                    store_point(u, kc, cB, valB, p0b, zB[H], vb, ix, vo, row_off(zbr), gvo, (zbr - pml) * nx4, rg,
                                 rec_row, row_inner && inner_x);
                 __builtin_amdgcn_wave_barrier();
             }

@@ -112,3 +112,26 @@ def test_c0_plain_and_batch(oracle_mod):
     _report("c0 batch ckpt=100", sb[0], syn_o, gb[0], grad_o)
     assert rel_err(mb[0], mis_o) <= MISFIT_RTOL
     assert rel_l2(gb[0], grad_o) <= GRAD_RTOL
+
+
+@pytest.mark.parametrize("lc,k,ckpt", [(3, 2, 0), (3, 1, 40), (6, 2, 0), (4, 3, 50)])
+def test_side_band_items(oracle_mod, lc, k, ckpt):
+    """Grids wide enough for side-band work items (band width 16 for Lc 3-4,
+    32 for Lc 6), with sources from the first to the last inner column and a
+    receiver on every inner column, so injection and records also happen in
+    band-item columns."""
+    nx, ns = 100, 3
+    p, vt, vs = case("small", seed=13, nx=nx, nz=30, Lc=lc, nt_interval=k, nt=240, ns=ns, ns0=0,
+                     ds=(nx - 1) // (ns - 1), nr=nx, dr=1, nr0=0, depths=1, depthr=0)
+    obs = oracle_mod.forward(p, vt)
+    syn_o, grad_o = oracle_mod.fwi(p, vs, obs)
+    syn_p, grad_p = solver.fwi_plain(p, dev(vs), dev(obs))
+    vel = dev(np.stack([vs.reshape(-1), vt.reshape(-1)]))
+    syn_b, grad_b, mis_b = solver.fwi_batch(p, vel.view(2, -1), dev(obs), ckpt_interval=ckpt)
+    syn_p, grad_p, syn_b, grad_b, mis_b = host(syn_p), host(grad_p), host(syn_b), host(grad_b), host(mis_b)
+    _report(f"bands Lc={lc} k={k} ckpt={ckpt}", syn_b[0], syn_o, grad_b[0], grad_o)
+    assert np.array_equal(syn_p, syn_o)          # same float operations as the oracle
+    assert np.array_equal(syn_b[0], syn_o)
+    assert rel_l2(grad_p, grad_o) <= GRAD_RTOL
+    assert rel_l2(grad_b[0], grad_o) <= GRAD_RTOL
+    assert rel_err(mis_b[0], oracle_mod.misfit_f32(syn_o, obs)) <= MISFIT_RTOL

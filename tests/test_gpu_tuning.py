@@ -28,8 +28,10 @@ def host(t):
 
 @pytest.fixture(scope="module")
 def small(oracle_mod):
-    # odd nt: a two-step launch schedule ends every pass with a one-step macro
-    p, vt, vs = case("small", seed=11, nt=301, nt_interval=2)
+    # odd nt: a two-step launch schedule ends every pass with a one-step macro;
+    # 100 columns: one-step launches use side-band items, two-step launches
+    # do not, and both must give the same bits
+    p, vt, vs = case("small", seed=11, nt=301, nt_interval=2, nx=100, nr=50)
     obs = oracle_mod.forward(p, vt)
     syn_o, grad_o = oracle_mod.fwi(p, vs, obs)
     vel = np.stack([vs.reshape(-1), (0.5 * (vs + vt)).reshape(-1)])
