@@ -16,7 +16,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 PKG = os.path.dirname(HERE)
 REPO = os.path.dirname(PKG)
 LIB = os.path.join(PKG, "libgeopvi_fwi2d.so")
-SOURCES = ["fwi2d_kernels.hip", "fwi2d_capi.hip"]
+SOURCES = ["fwi2d_kernels.hip", "fwi2d_psm.hip", "fwi2d_capi.hip"]
 HEADERS = ["fwi2d_device.h", "fwi2d_step.h", "fwi2d_kernels.h"]
 ARCH = os.environ.get("GPVI_OFFLOAD_ARCH", "gfx950")
 
@@ -51,7 +51,7 @@ def build(verbose: bool = False) -> str:
         _run(cmd, verbose)
         objs.append(obj)
     tmp = LIB + ".tmp"
-    _run([_hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs], verbose)
+    _run([_hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs, "-lhipfft"], verbose)
     os.replace(tmp, LIB)
     return LIB
 

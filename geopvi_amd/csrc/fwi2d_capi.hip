@@ -2,6 +2,7 @@
 #include "geopvi_fwi2d.h"
 
 #include <hip/hip_runtime.h>
+#include <hipfft/hipfft.h>
 
 #include <algorithm>
 #include <cmath>
@@ -69,16 +70,17 @@ int fd_coeffs(int lc, float* fd)
 int validate(const gpvi_fwi2d_params* p)
 {
     if (!p) return fail(GPVI_ERR_INVALID, "params is NULL");
-    if (p->laplace_slover != 0)
-        return fail(GPVI_ERR_UNSUPPORTED, "laplace_slover=%d: only the finite-difference Laplacian (0) is built",
+    if (p->laplace_slover != 0 && p->laplace_slover != 1)
+        return fail(GPVI_ERR_UNSUPPORTED, "laplace_slover=%d: 0 (finite differences) or 1 (pseudo-spectral)",
                     p->laplace_slover);
-    if (p->Lc < 3 || p->Lc > 6) return fail(GPVI_ERR_UNSUPPORTED, "Lc=%d: supported 3..6", p->Lc);
+    const bool psm = p->laplace_slover == 1;  // Lc is not used by the pseudo-spectral Laplacian
+    if (!psm && (p->Lc < 3 || p->Lc > 6)) return fail(GPVI_ERR_UNSUPPORTED, "Lc=%d: supported 3..6", p->Lc);
     if (p->nx < 1 || p->nz < 1 || p->ns < 1 || p->nt < 1 || p->nr < 1 || p->nt_interval < 1 || p->pml0 < 0)
         return fail(GPVI_ERR_INVALID, "nx, nz, ns, nt, nr, nt_interval must be >= 1 and pml0 >= 0");
     if (!(p->dx > 0) || !(p->dz > 0) || !(p->dt > 0) || !std::isfinite(p->f0) || p->f0 == 0.f)
         return fail(GPVI_ERR_INVALID, "dx, dz, dt must be > 0 and f0 finite and non-zero");
     if (p->nr > 1 && p->dr < 1) return fail(GPVI_ERR_INVALID, "dr=%d: receivers would coincide", p->dr);
-    const int lc = p->Lc + 1, pml = p->pml0 + lc;
+    const int lc = psm ? 0 : p->Lc + 1, pml = p->pml0 + lc;
     const long ntx = (long)p->nx + 2 * pml, ntz = (long)p->nz + 2 * pml;
     if (ntx > (1 << 20) || ntz > (1 << 20)) return fail(GPVI_ERR_INVALID, "grid too large");
     const long s0 = (long)pml + p->ns0, s1 = s0 + (long)(p->ns - 1) * p->ds;
@@ -167,7 +169,7 @@ struct Geom {
 // 64 - 2*(lc-1) owned columns plus the first step's halo on each side.
 void make_geom(const gpvi_fwi2d_params* p, int variant, int tb, Geom* g)
 {
-    g->lc = p->Lc + 1;
+    g->lc = p->laplace_slover == 1 ? 0 : p->Lc + 1;  // the pseudo-spectral path has Lc = 0 (fdtd.c:75-81)
     g->pml = p->pml0 + g->lc;
     g->ntx = p->nx + 2 * g->pml;
     g->ntz = p->nz + 2 * g->pml;
@@ -180,7 +182,7 @@ void make_geom(const gpvi_fwi2d_params* p, int variant, int tb, Geom* g)
     g->variant = variant;
     variant_tile(variant, &g->tile_x, &g->tile_z);
     g->zseg = g->tile_z;
-    g->tb = (tb == 2) ? 2 : 1;
+    g->tb = (tb == 2 && g->lc > 0) ? 2 : 1;
     g->stripw = (g->tb == 2) ? kStripWidth - 2 * (g->lc - 1) : kStripWidth;
     g->tile_x = g->stripw;
     g->tiles_x = (g->ntx + g->tile_x - 1) / g->tile_x;
@@ -194,7 +196,7 @@ void make_geom(const gpvi_fwi2d_params* p, int variant, int tb, Geom* g)
     // as w_left | w_right << 8.
     g->band_w = 0;
     g->band_tz = 0;
-    if (g->tb == 1 && g->pml <= 32) {
+    if (g->tb == 1 && g->lc > 0 && g->pml <= 32) {
         int best = -1, best_waste = 1 << 30;
         for (int wl = 16; wl <= 32; wl += 16)
             for (int wr = 16; wr <= 32; wr += 16) {
@@ -272,6 +274,8 @@ Sched make_sched(const Geom& g, int ckpt_req)
 struct BatchLayout {
     float *vel, *wav, *fwd, *adj, *ck, *save, *gpriv, *zero;
     double* partial;
+    double2* fft;   // pseudo-spectral path: one complex double field per unit
+    float* ktab;    // pseudo-spectral path: |k|^2 per padded point
     UnitDesc* units;
     TileRef* tiles;
     StepCtl* ctl;
@@ -299,6 +303,9 @@ void batch_layout(const Geom& g, const Sched& sc, int nsample, bool tables, char
     L->save = b.take<float>((size_t)(U * L->save_stride));
     L->gpriv = b.take<float>((size_t)(U * L->grad_stride));
     L->partial = b.take<double>((size_t)nsample * misfit_blocks((int64_t)g.ns * g.nt * g.nr));
+    const int64_t ntp = (int64_t)g.ntx * g.ntz;
+    L->fft = g.lc == 0 ? b.take<double2>((size_t)(U * ntp)) : nullptr;
+    L->ktab = g.lc == 0 ? b.take<float>((size_t)ntp) : nullptr;
     if (tables) {
         L->units = b.take<UnitDesc>((size_t)U);
         L->tiles = b.take<TileRef>((size_t)(U * g.tpu));
@@ -579,6 +586,7 @@ static void plain_layout(const Geom& g, char* base, bool store, PlainLayout* L)
 size_t gpvi_fwi2d_plain_workspace_bytes(const gpvi_fwi2d_params* p)
 {
     if (validate(p)) return 0;
+    if (p->laplace_slover == 1) return gpvi_fwi2d_batch_workspace_bytes(p, 1, 0);  // runs as a batch of one
     Geom g;
     make_geom(p, kPlainVariant, 1, &g);
     PlainLayout L;
@@ -612,6 +620,11 @@ static int plain_run(const gpvi_fwi2d_params* p, const float* vel_inner, float* 
     if (rc) return rc;
     if (!vel_inner || !record_syn || (adjoint && (!record_obs || !grad)))
         return fail(GPVI_ERR_INVALID, "NULL buffer");
+    if (p->laplace_slover == 1)  // pseudo-spectral: a batch of one sample, shots together
+        return gpvi_fwi2d_batch(p, 1, vel_inner, adjoint ? record_obs : record_syn, record_syn,
+                                adjoint ? grad : nullptr, nullptr, run_fwi, 0,
+                                adjoint ? GPVI_PHASE_ALL : GPVI_PHASE_FORWARD, adjoint ? 0 : GPVI_FLAG_NO_STORE,
+                                workspace, workspace_bytes, stream);
     Geom g;
     make_geom(p, kPlainVariant, 1, &g);
     PlainLayout L;
@@ -668,6 +681,59 @@ size_t gpvi_fwi2d_batch_workspace_bytes(const gpvi_fwi2d_params* p, int nsample,
     return L.bytes;
 }
 
+// ---------------------------------------------------------------------------
+// Pseudo-spectral Laplacian path: batched double-precision 2-D transforms
+// (hipFFT plans cached per shape and device), one launch sequence per step.
+namespace {
+std::mutex g_psm_mu;
+std::map<std::tuple<int, int, int, int>, hipfftHandle> g_psm_plans;
+
+int psm_plan(int ntz, int ntx, int batch, hipfftHandle* out)
+{
+    const auto key = std::make_tuple(current_device(), ntz, ntx, batch);
+    auto it = g_psm_plans.find(key);
+    if (it != g_psm_plans.end()) {
+        *out = it->second;
+        return 0;
+    }
+    if (g_psm_plans.size() >= 16) {
+        HIP_TRY(hipDeviceSynchronize());
+        for (auto& kv : g_psm_plans) (void)hipfftDestroy(kv.second);
+        g_psm_plans.clear();
+    }
+    int n[2] = {ntz, ntx};
+    hipfftHandle plan;
+    if (hipfftPlanMany(&plan, 2, n, nullptr, 1, 0, nullptr, 1, 0, HIPFFT_Z2Z, batch) != HIPFFT_SUCCESS)
+        return fail(GPVI_ERR_HIP, "hipfftPlanMany(%d x %d, batch %d) failed", ntz, ntx, batch);
+    g_psm_plans[key] = plan;
+    *out = plan;
+    return 0;
+}
+
+int psm_run(hipStream_t st, const Geom& g, const JobDesc& j, const BatchLayout& L, int nunit)
+{
+    const int ntp = g.ntx * g.ntz;
+    if (nunit > 65535) return fail(GPVI_ERR_INVALID, "pseudo-spectral batches hold at most 65535 units");
+    HIP_TRY(launch_psm_wavenumber(st, L.ktab, g.ntx, g.ntz, j.dx, j.dz));
+    std::lock_guard<std::mutex> lk(g_psm_mu);
+    hipfftHandle plan;
+    int rc;
+    if ((rc = psm_plan(g.ntz, g.ntx, nunit, &plan))) return rc;
+    if (hipfftSetStream(plan, st) != HIPFFT_SUCCESS) return fail(GPVI_ERR_HIP, "hipfftSetStream failed");
+    auto* z = reinterpret_cast<hipfftDoubleComplex*>(L.fft);
+    for (int s = 0; s < j.total; ++s) {
+        HIP_TRY(launch_psm_pack(st, L.units, nunit, s, L.fft, ntp));
+        if (hipfftExecZ2Z(plan, z, z, HIPFFT_FORWARD) != HIPFFT_SUCCESS)
+            return fail(GPVI_ERR_HIP, "hipfftExecZ2Z (forward) failed");
+        HIP_TRY(launch_psm_scale(st, L.fft, L.ktab, ntp, nunit));
+        if (hipfftExecZ2Z(plan, z, z, HIPFFT_BACKWARD) != HIPFFT_SUCCESS)
+            return fail(GPVI_ERR_HIP, "hipfftExecZ2Z (backward) failed");
+        HIP_TRY(launch_psm_point(st, L.units, nunit, s, L.fft, ntp));
+    }
+    return 0;
+}
+}  // namespace
+
 int gpvi_fwi2d_batch(const gpvi_fwi2d_params* p, int nsample, const float* vel_inner, const float* record_obs,
                      float* record_syn, float* grad, double* misfit, int run_fwi, int ckpt_interval, int phase,
                      int flags, void* workspace, size_t workspace_bytes, void* stream)
@@ -694,6 +760,15 @@ int gpvi_fwi2d_batch(const gpvi_fwi2d_params* p, int nsample, const float* vel_i
     const int store = (phase == GPVI_PHASE_FORWARD && (flags & GPVI_FLAG_NO_STORE)) ? 0 : 1;
     JobDesc j = make_job(p, g, sc, nsample, L, record_obs, record_syn, run_fwi, phase, store);
     HIP_TRY(launch_build_units(st, j, L.units, L.tiles));
+    if (g.lc == 0) {
+        if ((rc = psm_run(st, g, j, L, nsample * g.ns))) return rc;
+        if (phase != GPVI_PHASE_FORWARD)
+            HIP_TRY(launch_grad_reduce(st, L.gpriv, L.grad_stride, grad, g.ns, g.ni, nsample));
+        if (phase != GPVI_PHASE_ADJOINT && misfit)
+            HIP_TRY(launch_misfit(st, record_syn, record_obs, (int64_t)g.ns * g.nt * g.nr, nsample, L.partial,
+                                  misfit));
+        return 0;
+    }
     const int ntiles = nsample * g.ns * g.tpu;
     int n4[4] = {0, 0, 0, 0}, o4[4] = {0, 0, 0, 0};
     n4[g.lc - 4] = ntiles;
@@ -885,6 +960,9 @@ int gpvi_engine_submit(gpvi_engine* e, const gpvi_fwi2d_params* p, int nsample, 
     if (!e || !job_id) return fail(GPVI_ERR_INVALID, "NULL engine or job_id");
     int rc = validate(p);
     if (rc) return rc;
+    if (p->laplace_slover == 1)
+        return fail(GPVI_ERR_UNSUPPORTED, "the engine runs finite-difference jobs; pseudo-spectral ones go to "
+                                          "gpvi_fwi2d_batch");
     if (nsample < 1 || !vel_inner || !record_obs || !record_syn || !grad)
         return fail(GPVI_ERR_INVALID, "bad job arguments");
     EngineJob jb;

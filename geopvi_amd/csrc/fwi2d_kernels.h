@@ -64,6 +64,12 @@ hipError_t launch_build_units(hipStream_t st, const JobDesc& j, UnitDesc* units,
 hipError_t launch_grad_reduce(hipStream_t st, const float* gpriv, int64_t grad_stride, float* out, int ns,
                               int ni, int nsample);
 int misfit_blocks(int64_t len);
+
+// Pseudo-spectral Laplacian path (fwi2d_psm.hip).
+hipError_t launch_psm_wavenumber(hipStream_t st, float* k, int ntx, int ntz, float dx, float dz);
+hipError_t launch_psm_pack(hipStream_t st, const UnitDesc* units, int nunit, int s, double2* fft, int ntp);
+hipError_t launch_psm_scale(hipStream_t st, double2* fft, const float* k, int ntp, int nunit);
+hipError_t launch_psm_point(hipStream_t st, const UnitDesc* units, int nunit, int s, const double2* fft, int ntp);
 hipError_t launch_misfit(hipStream_t st, const float* syn, const float* obs, int64_t len, int nsample,
                          double* partial, double* misfit);
 

@@ -1,0 +1,163 @@
+// Pseudo-spectral Laplacian path (laplace_slover = 1) of the FWI-2D solver
+// for gfx950: geopvi/forward/fwi2d/fdtd2D_modelling.c with Lc = 0
+// (pml = pml0, fdtd.c:75-85) and pstd_2d_calculate_p (fdtd.c:783-830) in
+// place of the stencil.  One time step of every unit of a batch:
+//
+//   pack    p1 -> complex double field           (fdtd.c:791-799)
+//   FFT     batched 2-D forward transform         (hipFFT, Z2Z)
+//   scale   multiply by -|k|^2                    (fdtd.c:805-813)
+//   IFFT    batched 2-D inverse (unnormalised)    (hipFFT, Z2Z)
+//   point   p2 = 2 p1 - p0 + (v dt)^2 Re / (ntx ntz), then the four ordered
+//           absorbing sweeps over the pml0-wide bands, source or residual
+//           injection, store/records or imaging   (fdtd.c:819-826, 833-978)
+//
+// The transforms are double precision, as FFTW's fftw_complex is; every
+// other operation keeps the reference's float/double typing.
+#include "fwi2d_kernels.h"
+#include "fwi2d_step.h"
+
+namespace gpvi {
+
+constexpr double kPi = 3.1415926;  // the reference's pi (fdtd2D_modelling.c:15)
+
+// |k|^2 per padded point (wavenumber, fdtd.c:1127-1184): float dkx/dkz from
+// double expressions, float kx/kz, double squares summed and rounded.
+__global__ void psm_wavenumber_kernel(float* __restrict__ k, int ntx, int ntz, float dx, float dz)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= ntx * ntz) return;
+    const int iz = i / ntx, ix = i - iz * ntx;
+    const float dkz = (float)(1.0 / ntz / (double)dz);
+    const float dkx = (float)(1.0 / ntx / (double)dx);
+    const int az = ntz / 2 + (ntz % 2), ax = ntx / 2 + (ntx % 2);
+    const float kz = (float)(2 * kPi * (double)dkz * (double)(iz < az ? iz : ntz - iz));
+    const float kx = (float)(2 * kPi * (double)dkx * (double)(ix < ax ? ix : ntx - ix));
+    k[i] = (float)((double)kx * (double)kx + (double)kz * (double)kz);
+}
+
+// fft[unit][iz][ix] = (p1, 0) of each unit's current step.
+__global__ void psm_pack_kernel(const UnitDesc* __restrict__ units, int s, double2* __restrict__ fft, int ntp)
+{
+    const int un = blockIdx.y;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= ntp) return;
+    const UnitDesc& u = units[un];
+    const int ls = s - u.start + u.s_off;  // the unit's local step
+    if (ls < u.s_off || ls >= u.s_off + u.total) return;
+    const StepInfo si = decode_step(u, ls);
+    if (si.mode == kIdle) return;
+    const FieldPtrs f = step_fields(u, si);
+    const int iz = i / u.ntx, ix = i - iz * u.ntx;
+    fft[(int64_t)un * ntp + i] = make_double2((double)f.p1[(int64_t)iz * u.pitch + ix], 0.0);
+}
+
+// fft *= -k (both components; the float k is widened to double).
+__global__ void psm_scale_kernel(double2* __restrict__ fft, const float* __restrict__ k, int ntp, int64_t n)
+{
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const double mk = (double)(-k[i % ntp]);
+        double2 c = fft[i];
+        c.x = mk * c.x;
+        c.y = mk * c.y;
+        fft[i] = c;
+    }
+}
+
+// Everything after the Laplacian, one thread per padded point of a unit.
+__global__ void psm_point_kernel(const UnitDesc* __restrict__ units, int s, const double2* __restrict__ fft, int ntp)
+{
+    const int un = blockIdx.y;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= ntp) return;
+    const UnitDesc& u = units[un];
+    const int ls = s - u.start + u.s_off;  // the unit's local step
+    if (ls < u.s_off || ls >= u.s_off + u.total) return;
+    const StepInfo si = decode_step(u, ls);
+    if (si.mode == kIdle) return;
+    const FieldPtrs f = step_fields(u, si);
+    const int64_t margin = (int64_t)kPadRows * u.pitch;
+    const StepCtx c = make_step(u, si.mode, si.it, si.seg, f.out, margin);
+    const PointConsts kc = point_consts(u);
+    const int ntx = kc.ntx, ntz = kc.ntz, pml = kc.pml, pitch4 = kc.pitch4;
+    const int iz = i / ntx, ix = i - iz * ntx;
+    // byte offset of (iz, ix) from the first margin row, split so that the
+    // inward reads (up to 4 rows or columns either way) stay >= 0
+    const int ro = 4 * pitch4 + 16;
+    const unsigned vo = (unsigned)((iz + kPadRows) * pitch4 + ix * 4 - ro);
+    const Rsrc rp1 = make_rsrc(f.p1 - margin);
+    const Rsrc rp0 = make_rsrc(f.p0 - margin);
+    const Rsrc rv = make_rsrc(u.vel - margin);
+    const float p1c = bload(rp1, vo, ro), p0c = bload(rp0, vo, ro), v = bload(rv, vo, ro);
+    const double re = fft[(int64_t)un * ntp + i].x;
+    const float vdt = v * kc.dt;
+    float val = (float)((double)(2.f * p1c - p0c) + (double)(vdt * vdt) * re / (double)ntp);
+
+    const bool top = iz < pml, bot = iz >= ntz - pml, lef = ix < pml, rig = ix >= ntx - pml;
+    if (top || bot || lef || rig) {
+        const float sc = div1_const(v * kc.dt, kc.dx, kc.rdx, kc.hi);
+        const float t11 = ((2.f - sc) * (1.f - sc)) / 2.f;
+        const float t12 = sc * (2.f - sc);
+        const float t13 = (sc * (sc - 1.f)) / 2.f;
+        const Rsrc rw = make_rsrc(u.wtab);
+        if (top)
+            val = abc_blend(bload(rw, (unsigned)iz * 4u, 0),
+                            abc_extrapolate(p1c, bload(rp1, vo, ro + pitch4), bload(rp1, vo, ro + 2 * pitch4), rp0,
+                                            vo, ro, pitch4, t11, t12, t13), val);
+        if (bot)
+            val = abc_blend(bload(rw, (unsigned)(ntz - iz - 1) * 4u, 0),
+                            abc_extrapolate(p1c, bload(rp1, vo, ro - pitch4), bload(rp1, vo, ro - 2 * pitch4), rp0,
+                                            vo, ro, -pitch4, t11, t12, t13), val);
+        if (lef)
+            val = abc_blend(bload(rw, (unsigned)ix * 4u, 0),
+                            abc_extrapolate(p1c, bload(rp1, vo, ro + 4), bload(rp1, vo, ro + 8), rp0, vo, ro, 4, t11,
+                                            t12, t13), val);
+        if (rig)
+            val = abc_blend(bload(rw, (unsigned)(ntx - ix - 1) * 4u, 0),
+                            abc_extrapolate(p1c, bload(rp1, vo, ro - 4), bload(rp1, vo, ro - 8), rp0, vo, ro, -4, t11,
+                                            t12, t13), val);
+    }
+    const float dt = kc.dt;
+    const bool rec_row = iz == u.r_iz;
+    if (c.mode != kAdj) {
+        if (iz == u.s_iz && ix == u.s_ix) val = val + G(u.wav)[c.it] * dt * dt * v * v;
+    } else if (rec_row) {
+        const int d = ix - u.r_ix0;
+        if (d >= 0 && d % u.dr == 0 && d / u.dr < u.nr) {
+            const int ri = c.it * u.nr + d / u.dr;
+            const float res = kc.run_fwi ? G(u.syn)[ri] - G(u.obs)[ri] : G(u.obs)[ri];
+            val = val + res * dt * dt * v * v;
+        }
+    }
+    const bool inner = iz >= pml && iz < ntz - pml && ix >= pml && ix < ntx - pml;
+    const unsigned gvo = inner ? (unsigned)(((iz - pml) * u.nx + (ix - pml)) * 4) : 0u;
+    store_point(u, kc, c, val, p0c, p1c, v, ix, vo, ro, gvo, 0, make_rsrc(u.grad), rec_row, inner);
+}
+
+hipError_t launch_psm_wavenumber(hipStream_t st, float* k, int ntx, int ntz, float dx, float dz)
+{
+    const int n = ntx * ntz;
+    hipLaunchKernelGGL(psm_wavenumber_kernel, dim3((n + 255) / 256), dim3(256), 0, st, k, ntx, ntz, dx, dz);
+    return hipGetLastError();
+}
+
+hipError_t launch_psm_pack(hipStream_t st, const UnitDesc* units, int nunit, int s, double2* fft, int ntp)
+{
+    hipLaunchKernelGGL(psm_pack_kernel, dim3((ntp + 255) / 256, nunit), dim3(256), 0, st, units, s, fft, ntp);
+    return hipGetLastError();
+}
+
+hipError_t launch_psm_scale(hipStream_t st, double2* fft, const float* k, int ntp, int nunit)
+{
+    const int64_t n = (int64_t)ntp * nunit;
+    const int blocks = (int)std::min<int64_t>((n + 255) / 256, 65536);
+    hipLaunchKernelGGL(psm_scale_kernel, dim3(blocks), dim3(256), 0, st, fft, k, ntp, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_psm_point(hipStream_t st, const UnitDesc* units, int nunit, int s, const double2* fft, int ntp)
+{
+    hipLaunchKernelGGL(psm_point_kernel, dim3((ntp + 255) / 256, nunit), dim3(256), 0, st, units, s, fft, ntp);
+    return hipGetLastError();
+}
+
+}  // namespace gpvi

@@ -161,6 +161,24 @@ __device__ __forceinline__ float abc_blend(float w, float h, float u)
 
 // ---------------------------------------------------------------------------
 // ---------------------------------------------------------------------------
+// Field buffers of a unit's step: the pass's 4-buffer set (forward or
+// adjoint) rotated by the pass-local step, with the zero field or (when a
+// segment is recomputed) that segment's checkpoint pair as pre-history.
+__device__ __forceinline__ FieldPtrs step_fields(const UnitDesc& u, const StepInfo& si)
+{
+    const int64_t fe = u.field_elems;
+    const float* pre0 = u.zero;
+    const float* pre1 = u.zero;
+    float* base = u.fwd;
+    if (si.mode == kAdj) {
+        base = u.adj;
+    } else if (si.mode == kRec && si.seg > 0) {
+        pre0 = u.ck + (int64_t)(2 * si.seg) * fe;
+        pre1 = pre0 + fe;
+    }
+    return rotate_ptrs(base, fe, pre0, pre1, si.q);
+}
+
 // One time step of a macro, as seen by the points a wave updates.
 struct StepCtx {
     int it, mode;
@@ -430,16 +448,8 @@ __device__ __forceinline__ void process_item(const UnitDesc& u, int s, int item,
     const int ntx = u.ntx, ntz = u.ntz, pitch = u.pitch, pml = u.pml;
     const int64_t fe = u.field_elems;
     const int64_t margin = (int64_t)kPadRows * pitch;
-    const float* pre0 = u.zero;
-    const float* pre1 = u.zero;
-    float* base = u.fwd;
-    if (si.mode == kAdj) {
-        base = u.adj;
-    } else if (si.mode == kRec && si.seg > 0) {
-        pre0 = u.ck + (int64_t)(2 * si.seg) * fe;
-        pre1 = pre0 + fe;
-    }
-    const FieldPtrs f = rotate_ptrs(base, fe, pre0, pre1, si.q);
+    float* base = si.mode == kAdj ? u.adj : u.fwd;
+    const FieldPtrs f = step_fields(u, si);
     const StepCtx cA = make_step(u, si.mode, si.it, si.seg, f.out, margin);
     const int itB = si.mode == kAdj ? si.it - 1 : si.it + 1;
     StepCtx cB;

@@ -27,7 +27,7 @@ extern "C" {
 
 #define GPVI_OK 0
 #define GPVI_ERR_INVALID (-1)     /* bad argument or parameter value */
-#define GPVI_ERR_UNSUPPORTED (-2) /* laplace_slover=1 (PSM) or Lc outside 3..6 */
+#define GPVI_ERR_UNSUPPORTED (-2) /* laplace_slover not 0/1, Lc outside 3..6, PSM job to the engine */
 #define GPVI_ERR_GEOMETRY (-3)    /* a source or receiver outside the padded grid */
 #define GPVI_ERR_IO (-4)          /* parameter file missing or malformed */
 #define GPVI_ERR_HIP (-5)         /* HIP runtime failure */
@@ -63,9 +63,14 @@ int gpvi_abi_version(void);
  * (fdtd2D_modelling.c:1328-1379); returns GPVI_ERR_IO instead of exit(0). */
 int gpvi_fwi2d_read_params(const char *path, gpvi_fwi2d_params *out);
 
-/* Check that the FD path can run these parameters (no counterpart in the
- * reference, which has no checks): Lc in 3..6, laplace_slover == 0, sizes
- * positive, every source and receiver inside the padded grid. */
+/* Check that these parameters can run (no counterpart in the reference,
+ * which has no checks): laplace_slover 0 (finite differences, Lc in 3..6) or
+ * 1 (pseudo-spectral Laplacian, Lc unused and pml = pml0 as in
+ * fdtd2D_modelling.c:75-85), sizes positive, every source and receiver
+ * inside the padded grid.  Every entry point below runs both Laplacians
+ * except the engine, which takes finite-difference jobs only; a
+ * pseudo-spectral job runs as one batched call (hipFFT double-precision
+ * transforms, pstd_2d_calculate_p fdtd2D_modelling.c:783-830). */
 int gpvi_fwi2d_validate(const gpvi_fwi2d_params *p);
 
 /* ---- Plain solver: one sample, shots in sequence, one launch per step ----

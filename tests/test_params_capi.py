@@ -78,7 +78,7 @@ def test_param_file_blank_lines_and_errors(tmp_path):
 
 
 @pytest.mark.parametrize("change,code", [
-    (dict(laplace_slover=1), -2),
+    (dict(laplace_slover=2), -2),
     (dict(Lc=7), -2),
     (dict(Lc=2), -2),
     (dict(ns0=500), -3),
@@ -95,6 +95,17 @@ def test_validation_codes(change, code):
     assert _lib.last_error()
     with pytest.raises(_lib.GpviError):
         solver.validate(p)
+
+
+def test_pseudo_spectral_parameters_accepted():
+    # laplace_slover = 1 drops the stencil half-width from the padding
+    # (pml = pml0), so Lc is not checked and a source may sit closer to the edge
+    p = synth.workload_params("c0", laplace_slover=1, Lc=9)
+    assert _lib.load().gpvi_fwi2d_validate(ctypes.byref(p.to_c())) == 0
+    q = synth.workload_params("c0", laplace_slover=1, ns0=-p.pml0)
+    assert _lib.load().gpvi_fwi2d_validate(ctypes.byref(q.to_c())) == 0
+    q = synth.workload_params("c0", laplace_slover=1, ns0=-p.pml0 - 1)
+    assert _lib.load().gpvi_fwi2d_validate(ctypes.byref(q.to_c())) == -3
 
 
 def test_workspace_sizes():
