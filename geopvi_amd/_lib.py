@@ -32,6 +32,7 @@ SIGNATURES = {
     "gpvi_fwi2d_fwi_device": (_i, [_P, _vp, _vp, _vp, _vp, _i, _vp, _sz, _vp]),
     "gpvi_fwi2d_forward_device": (_i, [_P, _vp, _vp, _vp, _sz, _vp]),
     "gpvi_fwi2d_batch_workspace_bytes": (_sz, [_P, _i, _i]),
+    "gpvi_fwi2d_model_workspace_bytes": (_sz, [_P, _i]),
     "gpvi_fwi2d_batch": (_i, [_P, _i, _vp, _vp, _vp, _vp, _vp, _i, _i, _i, _i, _vp, _sz, _vp]),
     "gpvi_fwi2d_set_tuning": (_i, [_i, _i, _i]),
     "gpvi_fwi2d_get_tuning": (_i, [ctypes.POINTER(_i)] * 3),

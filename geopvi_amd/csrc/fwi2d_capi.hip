@@ -283,23 +283,26 @@ struct BatchLayout {
     size_t bytes;
 };
 
-void batch_layout(const Geom& g, const Sched& sc, int nsample, bool tables, char* base, BatchLayout* L)
+// model_only: forward modelling without a wavefield store (records only);
+// no adjoint fields, checkpoints, store or private gradients are laid out.
+void batch_layout(const Geom& g, const Sched& sc, int nsample, bool tables, char* base, BatchLayout* L,
+                  bool model_only = false)
 {
     Bump b{base};
     const int64_t U = (int64_t)nsample * g.ns;
     auto rnd = [](int64_t n) { return (n + 63) / 64 * 64; };  // keep per-unit blocks 256-B aligned
     L->fwd_stride = rnd(4 * g.fe);
-    L->adj_stride = rnd(4 * g.fe);
-    L->ck_stride = sc.nseg > 1 ? rnd(2 * (int64_t)sc.nseg * g.fe) : 0;
-    L->save_stride = rnd((int64_t)sc.nslot * g.ni);
-    L->grad_stride = rnd(g.ni);
+    L->adj_stride = model_only ? 0 : rnd(4 * g.fe);
+    L->ck_stride = (sc.nseg > 1 && !model_only) ? rnd(2 * (int64_t)sc.nseg * g.fe) : 0;
+    L->save_stride = model_only ? 0 : rnd((int64_t)sc.nslot * g.ni);
+    L->grad_stride = model_only ? 0 : rnd(g.ni);
     auto org = [&](float* p) { return p ? p + g.origin : nullptr; };
     L->vel = org(b.take<float>((size_t)nsample * g.fe));
     L->zero = org(b.take<float>((size_t)g.fe));
     L->wav = b.take<float>(g.nt + g.pml0 + 1);  // wavelet, then the ABC taper table
     L->fwd = org(b.take<float>((size_t)(U * L->fwd_stride)));
     L->adj = org(b.take<float>((size_t)(U * L->adj_stride)));
-    L->ck = sc.nseg > 1 ? org(b.take<float>((size_t)(U * L->ck_stride))) : nullptr;
+    L->ck = L->ck_stride ? org(b.take<float>((size_t)(U * L->ck_stride))) : nullptr;
     L->save = b.take<float>((size_t)(U * L->save_stride));
     L->gpriv = b.take<float>((size_t)(U * L->grad_stride));
     L->partial = b.take<double>((size_t)nsample * misfit_blocks((int64_t)g.ns * g.nt * g.nr));
@@ -681,6 +684,17 @@ size_t gpvi_fwi2d_batch_workspace_bytes(const gpvi_fwi2d_params* p, int nsample,
     return L.bytes;
 }
 
+size_t gpvi_fwi2d_model_workspace_bytes(const gpvi_fwi2d_params* p, int nsample)
+{
+    if (validate(p) || nsample < 1) return 0;
+    Geom g;
+    make_geom(p, tuning().variant, tuning().tb, &g);
+    Sched sc = make_sched(g, 0);
+    BatchLayout L;
+    batch_layout(g, sc, nsample, true, nullptr, &L, true);
+    return L.bytes;
+}
+
 // ---------------------------------------------------------------------------
 // Pseudo-spectral Laplacian path: batched double-precision 2-D transforms
 // (hipFFT plans cached per shape and device), one launch sequence per step.
@@ -748,11 +762,12 @@ int gpvi_fwi2d_batch(const gpvi_fwi2d_params* p, int nsample, const float* vel_i
     const Tuning tn = tuning();
     Geom g;
     make_geom(p, tn.variant, tn.tb, &g);
-    Sched sc = make_sched(g, ckpt_interval);
+    const bool model_only = phase == GPVI_PHASE_FORWARD && (flags & GPVI_FLAG_NO_STORE);
+    Sched sc = make_sched(g, model_only ? 0 : ckpt_interval);
     BatchLayout L;
-    batch_layout(g, sc, nsample, true, nullptr, &L);
+    batch_layout(g, sc, nsample, true, nullptr, &L, model_only);
     if ((rc = check_workspace(workspace, workspace_bytes, L.bytes))) return rc;
-    batch_layout(g, sc, nsample, true, (char*)workspace, &L);
+    batch_layout(g, sc, nsample, true, (char*)workspace, &L, model_only);
     hipStream_t st = (hipStream_t)stream;
     if (phase != GPVI_PHASE_ADJOINT) {
         if ((rc = job_setup(st, p, g, vel_inner, L.vel, L.wav, L.zero, nsample))) return rc;

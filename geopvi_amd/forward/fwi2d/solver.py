@@ -70,6 +70,16 @@ def batch_workspace_bytes(params: FWIParams, nsample: int, ckpt_interval: int = 
     return int(n)
 
 
+def model_workspace_bytes(params: FWIParams, nsample: int) -> int:
+    """Workspace of forward modelling only (fields and records, no store)."""
+    c, pc = _params_ptr(params)
+    n = _lib.load().gpvi_fwi2d_model_workspace_bytes(pc, int(nsample))
+    if n == 0:
+        validate(params)
+        raise ValueError(f"nsample={nsample}")
+    return int(n)
+
+
 def schedule(params: FWIParams, ckpt_interval: int = 0, steps_per_launch: int = 1) -> dict:
     """Adjoint schedule and launch counts of a batched evaluation.
 
@@ -179,7 +189,7 @@ def forward_batch(params: FWIParams, vel: torch.Tensor, graph: bool = True) -> t
     dev = vel.device
     syn = torch.empty(nsample, params.record_len, dtype=torch.float32, device=dev)
     dummy_obs = torch.zeros(params.record_len, dtype=torch.float32, device=dev)
-    ws = _workspace(batch_workspace_bytes(params, nsample, 0), dev)
+    ws = _workspace(model_workspace_bytes(params, nsample), dev)
     flags = _lib.GPVI_FLAG_NO_STORE | (0 if graph else _lib.GPVI_FLAG_NO_GRAPH)
     _batch_call(params, nsample, vel, dummy_obs, syn, None, None, 1, 0, _lib.GPVI_PHASE_FORWARD, flags, ws, dev)
     return syn

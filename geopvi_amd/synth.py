@@ -128,6 +128,19 @@ def gaussian_samples(mean: np.ndarray, std: float, n: int, seed: int) -> np.ndar
     return flat[None, :] + std * rng.standard_normal((n, flat.size))
 
 
+def edge_taper(nz: int, nx: int, width: int) -> np.ndarray:
+    """(nz, nx) weights: 0 on the outermost row and column, rising as
+    sin^2 to 1 over ``width`` cells, 1 inside.  Multiplying a model
+    perturbation by it leaves the edge-replicated absorbing bands unchanged."""
+    def t(n):
+        w = np.ones(n)
+        r = np.sin(0.5 * np.pi * np.arange(width) / width) ** 2
+        w[:width] = r
+        w[n - width:] = np.minimum(w[n - width:], r[::-1])
+        return w
+    return t(nz)[:, None] * t(nx)[None, :]
+
+
 def add_noise(data: np.ndarray, frac: float, seed: int) -> np.ndarray:
     """Additive Gaussian noise with std = frac * RMS(data)."""
     rng = np.random.default_rng(seed)

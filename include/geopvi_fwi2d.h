@@ -102,6 +102,10 @@ int gpvi_fwi2d_forward_device(const gpvi_fwi2d_params *p, const float *vel_inner
  * segment is recomputed during the adjoint.  misfit may be NULL; grad may be
  * NULL in GPVI_PHASE_FORWARD. */
 size_t gpvi_fwi2d_batch_workspace_bytes(const gpvi_fwi2d_params *p, int nsample, int ckpt_interval);
+/* Workspace of a forward-modelling-only call (GPVI_PHASE_FORWARD with
+ * GPVI_FLAG_NO_STORE; the reference's forward_2D, fdtd2D_modelling.c:356-675,
+ * allocates a wavefield store it never reads): fields and records only. */
+size_t gpvi_fwi2d_model_workspace_bytes(const gpvi_fwi2d_params *p, int nsample);
 int gpvi_fwi2d_batch(const gpvi_fwi2d_params *p, int nsample, const float *vel_inner,
                      const float *record_obs, float *record_syn, float *grad, double *misfit,
                      int run_fwi, int ckpt_interval, int phase, int flags, void *workspace,

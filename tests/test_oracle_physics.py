@@ -94,15 +94,20 @@ def test_gradient_against_finite_differences(oracle_mod):
     """Directional derivatives of the misfit vs <-grad, d> (posterior.py:110
     returns -grad as the misfit gradient).
 
-    The reference's adjoint is a continuous one: the same propagator runs
-    backwards with the residual injected as res*dt^2*v_r^2 (fdtd.c:955) and
-    the image is -2/v^3 * u_tt * q * k * dt (fdtd.c:965); the ABC is not
-    adjointed and the v-dependence of the source term is dropped.  Against
-    the exact derivative 2*dt^2/v * u_tt * lambda this is a positive,
-    depth-varying scale of about v_r^2*dt/v^2.  So the check is on the
-    direction: after the best positive scale the directional derivatives
-    agree within 5 %, their cosine exceeds 0.99, and the scale lies in the
-    range the formula predicts."""
+    The reference runs the forward propagator backwards with the residual
+    injected as res*dt^2*v_r^2 (fdtd.c:955) and images -2/v^3 * u_tt * q *
+    k * dt (fdtd.c:965).  With u^{n+1} = 2u^n - u^{n-1} + dt^2 V L u^n the
+    exact adjoint lambda satisfies the same recursion for mu = V lambda, so
+    q = dt^2 v^2 lambda and the exact derivative sum_n 2 dt^2/v u_tt lambda
+    equals -grad / dt: one scale, 1/dt, in the interior.  Two parts of the
+    discrete misfit are outside the image: the ABC, which is not adjointed,
+    and the absorbing bands' velocities, which are edge-replicated copies of
+    the model's outermost rows and columns (fdtd.c:1197-1238) -- a
+    perturbation of an edge cell moves a whole line of band cells next to the
+    surface sources and receivers.  So the directions vanish on the model's
+    edge, and after the best positive scale the directional derivatives
+    agree within 5 %, their cosine exceeds 0.99, and the scale is within a
+    factor 2 of 1/dt."""
     p = synth.workload_params("tiny", nt_interval=1)
     vt = synth.gradient_anomaly_model(p.nz, p.nx, seed=6)
     vs = synth.smooth(vt, 3.0)
@@ -117,7 +122,7 @@ def test_gradient_against_finite_differences(oracle_mod):
     rng = np.random.default_rng(11)
     fd, an = [], []
     for _ in range(4):
-        d = synth.smooth(rng.standard_normal((p.nz, p.nx)), 2.0)
+        d = synth.smooth(rng.standard_normal((p.nz, p.nx)), 2.0) * synth.edge_taper(p.nz, p.nx, 3)
         d /= np.abs(d).max()
         eps = 1.0  # m/s
         fd.append((misfit(vs + eps * d) - misfit(vs - eps * d)) / (2 * eps))
@@ -128,6 +133,5 @@ def test_gradient_against_finite_differences(oracle_mod):
     rel = float(np.linalg.norm(fd - alpha * an) / np.linalg.norm(fd))
     print(f"FD check: cos={cos:.5f} scale={alpha:.1f} rel-after-scale={rel:.3e}")
     dt = float(np.float32(p.dt))
-    vmin, vmax = float(vs.min()), float(vs.max())
     assert cos > 0.99 and rel < 0.05
-    assert (vmin / vmax) ** 2 / dt < alpha < (vmax / vmin) ** 2 / dt
+    assert 0.5 / dt < alpha < 2.0 / dt

@@ -116,6 +116,10 @@ def test_workspace_sizes():
     assert full > 16 * 20 * p.nsave * p.ni * 4
     assert ck < full / 5
     assert solver.plain_workspace_bytes(p) > p.nsave * p.ni * 4
+    # forward modelling only: four rotating fields per unit, no store
+    model = solver.model_workspace_bytes(p, 16)
+    fe = (p.nz + 2 * (p.pml0 + p.Lc + 1) + 32) * 448
+    assert 16 * 20 * 4 * fe * 4 <= model < 16 * 20 * 5 * fe * 4 + 16 * fe * 4 + (1 << 24)
 
 
 def test_tuning_round_trip():
