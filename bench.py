@@ -59,6 +59,8 @@ def parse():
     ap.add_argument("--cpu-shots", type=int, default=20)
     ap.add_argument("--no-vi", action="store_true", help="skip the end-to-end inversion leg")
     ap.add_argument("--vi-iterations", type=int, default=2)
+    ap.add_argument("--no-psm", action="store_true", help="skip the pseudo-spectral (laplace_slover=1) leg")
+    ap.add_argument("--psm-nsample", type=int, default=4)
     ap.add_argument("--seed", type=int, default=1)
     return ap.parse_args()
 
@@ -181,6 +183,38 @@ def vi_leg(args, dev):
         return out
     finally:
         torch.set_default_dtype(old)
+
+
+def psm_leg(args, dev, p_fd, samples, obs):
+    """The pseudo-spectral Laplacian (laplace_slover = 1) on the c1 geometry:
+    `psm_nsample` samples x 20 shots batched (one hipFFT plan over every
+    unit, double-precision transforms), and one sample alone through the
+    plain entry point.  One untimed warm-up evaluation each."""
+    import torch
+    from geopvi_amd.forward.fwi2d import solver
+
+    p = p_fd.replace(laplace_slover=1)
+    n = max(1, min(args.psm_nsample, len(samples)))
+    vel = torch.from_numpy(np.ascontiguousarray(samples[:n], np.float32)).to(dev)
+
+    def timed(fn):
+        fn()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        fn()
+        torch.cuda.synchronize()
+        return time.perf_counter() - t0
+
+    t_batch = timed(lambda: solver.fwi_batch(p, vel, obs))
+    t_plain = timed(lambda: solver.fwi_plain(p, vel[0], obs))
+    ntx, ntz = p.nx + 2 * p.pml0, p.nz + 2 * p.pml0
+    out = {"grid_padded": [ntz, ntx], "nsample": n, "shots": p.ns, "nt": p.nt,
+           "batched_samples_per_s": round(n / t_batch, 4), "plain_samples_per_s": round(1 / t_plain, 4),
+           "speedup_vs_plain": round(t_plain * n / t_batch, 2),
+           "us_per_time_step_batched": round(t_batch / (2 * p.nt) * 1e6, 1),
+           "fft": "hipFFT D2Z + Z2D 2-D in place, double, batch = samples x shots, one pair per time step"}
+    log(f"psm: batched {n / t_batch:.3f} samples/s, plain {1 / t_plain:.3f} samples/s")
+    return out
 
 
 def main():
@@ -334,6 +368,10 @@ def main():
 
     if not args.no_vi:
         out["vi"] = vi_leg(args, dev)
+        torch.cuda.empty_cache()
+
+    if not args.no_psm:
+        out["psm"] = psm_leg(args, dev, p, samples, obs)
         torch.cuda.empty_cache()
 
     if not args.no_cpu:

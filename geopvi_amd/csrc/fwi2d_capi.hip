@@ -274,7 +274,7 @@ Sched make_sched(const Geom& g, int ckpt_req)
 struct BatchLayout {
     float *vel, *wav, *fwd, *adj, *ck, *save, *gpriv, *zero;
     double* partial;
-    double2* fft;   // pseudo-spectral path: one complex double field per unit
+    double* fft;    // pseudo-spectral path: per unit, ntz rows of 2 (ntx/2 + 1) doubles (in-place r2c/c2r)
     float* ktab;    // pseudo-spectral path: |k|^2 per padded point
     UnitDesc* units;
     TileRef* tiles;
@@ -307,7 +307,7 @@ void batch_layout(const Geom& g, const Sched& sc, int nsample, bool tables, char
     L->gpriv = b.take<float>((size_t)(U * L->grad_stride));
     L->partial = b.take<double>((size_t)nsample * misfit_blocks((int64_t)g.ns * g.nt * g.nr));
     const int64_t ntp = (int64_t)g.ntx * g.ntz;
-    L->fft = g.lc == 0 ? b.take<double2>((size_t)(U * ntp)) : nullptr;
+    L->fft = g.lc == 0 ? b.take<double>((size_t)(U * g.ntz * 2 * (g.ntx / 2 + 1))) : nullptr;
     L->ktab = g.lc == 0 ? b.take<float>((size_t)ntp) : nullptr;
     if (tables) {
         L->units = b.take<UnitDesc>((size_t)U);
@@ -697,12 +697,17 @@ size_t gpvi_fwi2d_model_workspace_bytes(const gpvi_fwi2d_params* p, int nsample)
 
 // ---------------------------------------------------------------------------
 // Pseudo-spectral Laplacian path: batched double-precision 2-D transforms
-// (hipFFT plans cached per shape and device), one launch sequence per step.
+// (hipFFT plan pairs cached per shape and device), one launch sequence per
+// step.  In place: the real field of a unit has rows of rp = 2 (ntx/2 + 1)
+// doubles, its spectrum ntx/2 + 1 complex values per row.
 namespace {
+struct PsmPlans {
+    hipfftHandle fwd, inv;  // D2Z, Z2D
+};
 std::mutex g_psm_mu;
-std::map<std::tuple<int, int, int, int>, hipfftHandle> g_psm_plans;
+std::map<std::tuple<int, int, int, int>, PsmPlans> g_psm_plans;
 
-int psm_plan(int ntz, int ntx, int batch, hipfftHandle* out)
+int psm_plans(int ntz, int ntx, int batch, PsmPlans* out)
 {
     const auto key = std::make_tuple(current_device(), ntz, ntx, batch);
     auto it = g_psm_plans.find(key);
@@ -712,37 +717,48 @@ int psm_plan(int ntz, int ntx, int batch, hipfftHandle* out)
     }
     if (g_psm_plans.size() >= 16) {
         HIP_TRY(hipDeviceSynchronize());
-        for (auto& kv : g_psm_plans) (void)hipfftDestroy(kv.second);
+        for (auto& kv : g_psm_plans) {
+            (void)hipfftDestroy(kv.second.fwd);
+            (void)hipfftDestroy(kv.second.inv);
+        }
         g_psm_plans.clear();
     }
+    const int nc = ntx / 2 + 1, rp = 2 * nc;
     int n[2] = {ntz, ntx};
-    hipfftHandle plan;
-    if (hipfftPlanMany(&plan, 2, n, nullptr, 1, 0, nullptr, 1, 0, HIPFFT_Z2Z, batch) != HIPFFT_SUCCESS)
-        return fail(GPVI_ERR_HIP, "hipfftPlanMany(%d x %d, batch %d) failed", ntz, ntx, batch);
-    g_psm_plans[key] = plan;
-    *out = plan;
+    int real_embed[2] = {ntz, rp}, spec_embed[2] = {ntz, nc};
+    PsmPlans pl;
+    if (hipfftPlanMany(&pl.fwd, 2, n, real_embed, 1, ntz * rp, spec_embed, 1, ntz * nc, HIPFFT_D2Z, batch) !=
+        HIPFFT_SUCCESS)
+        return fail(GPVI_ERR_HIP, "hipfftPlanMany(D2Z %d x %d, batch %d) failed", ntz, ntx, batch);
+    if (hipfftPlanMany(&pl.inv, 2, n, spec_embed, 1, ntz * nc, real_embed, 1, ntz * rp, HIPFFT_Z2D, batch) !=
+        HIPFFT_SUCCESS) {
+        (void)hipfftDestroy(pl.fwd);
+        return fail(GPVI_ERR_HIP, "hipfftPlanMany(Z2D %d x %d, batch %d) failed", ntz, ntx, batch);
+    }
+    g_psm_plans[key] = pl;
+    *out = pl;
     return 0;
 }
 
 int psm_run(hipStream_t st, const Geom& g, const JobDesc& j, const BatchLayout& L, int nunit)
 {
-    const int ntp = g.ntx * g.ntz;
     if (nunit > 65535) return fail(GPVI_ERR_INVALID, "pseudo-spectral batches hold at most 65535 units");
     HIP_TRY(launch_psm_wavenumber(st, L.ktab, g.ntx, g.ntz, j.dx, j.dz));
     std::lock_guard<std::mutex> lk(g_psm_mu);
-    hipfftHandle plan;
+    PsmPlans pl;
     int rc;
-    if ((rc = psm_plan(g.ntz, g.ntx, nunit, &plan))) return rc;
-    if (hipfftSetStream(plan, st) != HIPFFT_SUCCESS) return fail(GPVI_ERR_HIP, "hipfftSetStream failed");
-    auto* z = reinterpret_cast<hipfftDoubleComplex*>(L.fft);
+    if ((rc = psm_plans(g.ntz, g.ntx, nunit, &pl))) return rc;
+    if (hipfftSetStream(pl.fwd, st) != HIPFFT_SUCCESS || hipfftSetStream(pl.inv, st) != HIPFFT_SUCCESS)
+        return fail(GPVI_ERR_HIP, "hipfftSetStream failed");
+    auto* spec = reinterpret_cast<hipfftDoubleComplex*>(L.fft);
     for (int s = 0; s < j.total; ++s) {
-        HIP_TRY(launch_psm_pack(st, L.units, nunit, s, L.fft, ntp));
-        if (hipfftExecZ2Z(plan, z, z, HIPFFT_FORWARD) != HIPFFT_SUCCESS)
-            return fail(GPVI_ERR_HIP, "hipfftExecZ2Z (forward) failed");
-        HIP_TRY(launch_psm_scale(st, L.fft, L.ktab, ntp, nunit));
-        if (hipfftExecZ2Z(plan, z, z, HIPFFT_BACKWARD) != HIPFFT_SUCCESS)
-            return fail(GPVI_ERR_HIP, "hipfftExecZ2Z (backward) failed");
-        HIP_TRY(launch_psm_point(st, L.units, nunit, s, L.fft, ntp));
+        HIP_TRY(launch_psm_pack(st, L.units, nunit, s, L.fft, g.ntx, g.ntz));
+        if (hipfftExecD2Z(pl.fwd, L.fft, spec) != HIPFFT_SUCCESS)
+            return fail(GPVI_ERR_HIP, "hipfftExecD2Z failed");
+        HIP_TRY(launch_psm_scale(st, L.fft, L.ktab, g.ntx, g.ntz, nunit));
+        if (hipfftExecZ2D(pl.inv, spec, L.fft) != HIPFFT_SUCCESS)
+            return fail(GPVI_ERR_HIP, "hipfftExecZ2D failed");
+        HIP_TRY(launch_psm_point(st, L.units, nunit, s, L.fft, g.ntx, g.ntz));
     }
     return 0;
 }

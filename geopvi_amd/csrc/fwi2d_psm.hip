@@ -3,16 +3,24 @@
 // (pml = pml0, fdtd.c:75-85) and pstd_2d_calculate_p (fdtd.c:783-830) in
 // place of the stencil.  One time step of every unit of a batch:
 //
-//   pack    p1 -> complex double field           (fdtd.c:791-799)
-//   FFT     batched 2-D forward transform         (hipFFT, Z2Z)
+//   pack    p1 -> double field                    (fdtd.c:791-799)
+//   FFT     batched 2-D forward transform         (hipFFT, D2Z)
 //   scale   multiply by -|k|^2                    (fdtd.c:805-813)
-//   IFFT    batched 2-D inverse (unnormalised)    (hipFFT, Z2Z)
+//   IFFT    batched 2-D inverse (unnormalised)    (hipFFT, Z2D)
 //   point   p2 = 2 p1 - p0 + (v dt)^2 Re / (ntx ntz), then the four ordered
 //           absorbing sweeps over the pml0-wide bands, source or residual
 //           injection, store/records or imaging   (fdtd.c:819-826, 833-978)
 //
-// The transforms are double precision, as FFTW's fftw_complex is; every
-// other operation keeps the reference's float/double typing.
+// The reference transforms the real field as a complex one (fftw c2c) and
+// keeps the real part of the inverse.  The spectrum of a real field is
+// Hermitian and |k|^2 is even in both axes (kx(ix) = kx(ntx - ix)), so the
+// scaled spectrum stays Hermitian and its inverse is real: the
+// real-to-complex / complex-to-real pair computes the same values with half
+// the transform work; only the rounding differs, as between any two FFT
+// libraries.  Transforms are in place, double precision (FFTW's
+// fftw_complex is double); real rows are padded to rp = 2 (ntx/2 + 1)
+// doubles, the spectrum has ntx/2 + 1 complex columns.  Every other
+// operation keeps the reference's float/double typing.
 #include "fwi2d_kernels.h"
 #include "fwi2d_step.h"
 
@@ -35,8 +43,9 @@ __global__ void psm_wavenumber_kernel(float* __restrict__ k, int ntx, int ntz, f
     k[i] = (float)((double)kx * (double)kx + (double)kz * (double)kz);
 }
 
-// fft[unit][iz][ix] = (p1, 0) of each unit's current step.
-__global__ void psm_pack_kernel(const UnitDesc* __restrict__ units, int s, double2* __restrict__ fft, int ntp)
+// re[unit][iz][ix] (row pitch rp) = p1 of each unit's current step.
+__global__ void psm_pack_kernel(const UnitDesc* __restrict__ units, int s, double* __restrict__ re, int rp,
+                                int64_t ustride, int ntp)
 {
     const int un = blockIdx.y;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -48,23 +57,29 @@ __global__ void psm_pack_kernel(const UnitDesc* __restrict__ units, int s, doubl
     if (si.mode == kIdle) return;
     const FieldPtrs f = step_fields(u, si);
     const int iz = i / u.ntx, ix = i - iz * u.ntx;
-    fft[(int64_t)un * ntp + i] = make_double2((double)f.p1[(int64_t)iz * u.pitch + ix], 0.0);
+    re[un * ustride + (int64_t)iz * rp + ix] = (double)f.p1[(int64_t)iz * u.pitch + ix];
 }
 
-// fft *= -k (both components; the float k is widened to double).
-__global__ void psm_scale_kernel(double2* __restrict__ fft, const float* __restrict__ k, int ntp, int64_t n)
+// spectrum *= -k (both components; the float k is widened to double).  The
+// spectrum holds ntz rows of nc = ntx/2 + 1 columns per unit; column ix of
+// row iz takes k[iz][ix].
+__global__ void psm_scale_kernel(double2* __restrict__ spec, const float* __restrict__ k, int ntx, int nc,
+                                 int64_t per_unit, int64_t n)
 {
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-        const double mk = (double)(-k[i % ntp]);
-        double2 c = fft[i];
+        const int64_t j = i % per_unit;
+        const int iz = (int)(j / nc), ix = (int)(j - (int64_t)iz * nc);
+        const double mk = (double)(-k[(int64_t)iz * ntx + ix]);
+        double2 c = spec[i];
         c.x = mk * c.x;
         c.y = mk * c.y;
-        fft[i] = c;
+        spec[i] = c;
     }
 }
 
 // Everything after the Laplacian, one thread per padded point of a unit.
-__global__ void psm_point_kernel(const UnitDesc* __restrict__ units, int s, const double2* __restrict__ fft, int ntp)
+__global__ void psm_point_kernel(const UnitDesc* __restrict__ units, int s, const double* __restrict__ re_all,
+                                 int rp, int64_t ustride, int ntp)
 {
     const int un = blockIdx.y;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -88,7 +103,7 @@ __global__ void psm_point_kernel(const UnitDesc* __restrict__ units, int s, cons
     const Rsrc rp0 = make_rsrc(f.p0 - margin);
     const Rsrc rv = make_rsrc(u.vel - margin);
     const float p1c = bload(rp1, vo, ro), p0c = bload(rp0, vo, ro), v = bload(rv, vo, ro);
-    const double re = fft[(int64_t)un * ntp + i].x;
+    const double re = re_all[un * ustride + (int64_t)iz * rp + ix];
     const float vdt = v * kc.dt;
     float val = (float)((double)(2.f * p1c - p0c) + (double)(vdt * vdt) * re / (double)ntp);
 
@@ -140,23 +155,34 @@ hipError_t launch_psm_wavenumber(hipStream_t st, float* k, int ntx, int ntz, flo
     return hipGetLastError();
 }
 
-hipError_t launch_psm_pack(hipStream_t st, const UnitDesc* units, int nunit, int s, double2* fft, int ntp)
+// In-place real/complex buffer of one unit: ntz rows of rp = 2 (ntx/2 + 1)
+// doubles (real view) = ntx/2 + 1 complex values (spectrum view).
+static inline int psm_rp(int ntx) { return 2 * (ntx / 2 + 1); }
+
+hipError_t launch_psm_pack(hipStream_t st, const UnitDesc* units, int nunit, int s, double* buf, int ntx, int ntz)
 {
-    hipLaunchKernelGGL(psm_pack_kernel, dim3((ntp + 255) / 256, nunit), dim3(256), 0, st, units, s, fft, ntp);
+    const int ntp = ntx * ntz, rp = psm_rp(ntx);
+    hipLaunchKernelGGL(psm_pack_kernel, dim3((ntp + 255) / 256, nunit), dim3(256), 0, st, units, s, buf, rp,
+                       (int64_t)ntz * rp, ntp);
     return hipGetLastError();
 }
 
-hipError_t launch_psm_scale(hipStream_t st, double2* fft, const float* k, int ntp, int nunit)
+hipError_t launch_psm_scale(hipStream_t st, double* buf, const float* k, int ntx, int ntz, int nunit)
 {
-    const int64_t n = (int64_t)ntp * nunit;
+    const int nc = ntx / 2 + 1;
+    const int64_t per_unit = (int64_t)ntz * nc, n = per_unit * nunit;
     const int blocks = (int)std::min<int64_t>((n + 255) / 256, 65536);
-    hipLaunchKernelGGL(psm_scale_kernel, dim3(blocks), dim3(256), 0, st, fft, k, ntp, n);
+    hipLaunchKernelGGL(psm_scale_kernel, dim3(blocks), dim3(256), 0, st, reinterpret_cast<double2*>(buf), k, ntx,
+                       nc, per_unit, n);
     return hipGetLastError();
 }
 
-hipError_t launch_psm_point(hipStream_t st, const UnitDesc* units, int nunit, int s, const double2* fft, int ntp)
+hipError_t launch_psm_point(hipStream_t st, const UnitDesc* units, int nunit, int s, const double* buf, int ntx,
+                            int ntz)
 {
-    hipLaunchKernelGGL(psm_point_kernel, dim3((ntp + 255) / 256, nunit), dim3(256), 0, st, units, s, fft, ntp);
+    const int ntp = ntx * ntz, rp = psm_rp(ntx);
+    hipLaunchKernelGGL(psm_point_kernel, dim3((ntp + 255) / 256, nunit), dim3(256), 0, st, units, s, buf, rp,
+                       (int64_t)ntz * rp, ntp);
     return hipGetLastError();
 }
 
