@@ -115,7 +115,9 @@ def serve_stream(args, rate, nreq, seed, dev):
         t_end = max(r.finished for r in res)
     lat = np.array([r.latency for r in res])
     span = t_end - t0
-    return {"requests": nreq, "rate_req_s": rate, "p50_ms": round(float(np.percentile(lat, 50)) * 1e3, 1),
+    n_full = sum(1 for r in res if r.extra.get("ckpt_interval") == 0)
+    return {"requests": nreq, "rate_req_s": rate, "store": f"auto: {n_full} full store, {nreq - n_full} checkpointed",
+            "p50_ms": round(float(np.percentile(lat, 50)) * 1e3, 1),
             "p99_ms": round(float(np.percentile(lat, 99)) * 1e3, 1),
             "mean_ms": round(float(lat.mean()) * 1e3, 1), "makespan_s": round(span, 3),
             "samples_per_s": round(total_samples / span, 3),

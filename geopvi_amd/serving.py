@@ -38,10 +38,33 @@ class FWIResult:
     submitted: float
     admitted: float
     finished: float
+    extra: dict = field(default_factory=dict)  # "ckpt_interval": the store mode the job ran with (0 = full)
 
     @property
     def latency(self) -> float:
         return self.finished - self.submitted
+
+
+def choose_store(full_bytes: int, ckpt_bytes: int, in_use: int, budget: int, policy: str = "auto",
+                 full_share: int = 8) -> bool:
+    """True: run the job with the full wavefield store; False: checkpointed.
+
+    The full store skips the recompute pass (two passes over the time axis
+    instead of three) and gives identical results.  "auto" takes it when the
+    job's full-store footprint is at most 1/full_share of the budget and fits
+    in what is free now, so a few large jobs cannot crowd out concurrency;
+    "full" whenever it fits now; "checkpoint" never.  The caller has checked
+    that the checkpointed footprint fits."""
+    if policy not in ("auto", "full", "checkpoint"):
+        raise ValueError(f"store policy {policy!r}: 'auto', 'full' or 'checkpoint'")
+    if policy == "checkpoint":
+        return False
+    if full_bytes <= ckpt_bytes:  # checkpointing would save nothing
+        return True
+    fits = in_use + full_bytes <= budget
+    if policy == "full":
+        return fits
+    return fits and full_bytes * full_share <= budget
 
 
 @dataclass
@@ -50,7 +73,7 @@ class _Request:
     vel: torch.Tensor
     obs: torch.Tensor
     run_fwi: int
-    ckpt: int
+    ckpt: Optional[int]  # None: chosen at admission by the store policy
     need: int
     ready: torch.cuda.Event
     future: Future
@@ -64,20 +87,28 @@ class FWIServer:
     """Serve FWI-2D misfit+gradient requests with continuous batching.
 
     mem_budget: bytes of device memory admitted jobs may hold (default: 90 %
-    of the free memory when the server starts).  steps_per_tick: time steps
+    of the free memory when the server starts, counting blocks torch's
+    allocator holds cached; they go back to the device when the server
+    closes).  steps_per_tick: time steps
     advanced between admission/completion checks (rounded to whole graph
-    segments by the engine).
+    segments by the engine).  store_policy: how a request without an explicit
+    ckpt_interval keeps its forward wavefield (see choose_store); requests
+    are admitted by the footprint of the chosen mode.
     """
 
     def __init__(self, device: int = 0, mem_budget: Optional[int] = None, steps_per_tick: int = 64,
-                 max_units: int = 16384, max_tiles: int = 1 << 21):
+                 max_units: int = 16384, max_tiles: int = 1 << 21, store_policy: str = "auto"):
+        choose_store(0, 0, 0, 1, store_policy)  # validates the name
         self.device = torch.device("cuda", device)
         torch.cuda.set_device(self.device)
         self.stream = torch.cuda.Stream(self.device)
         if mem_budget is None:
+            # free device memory plus what torch's allocator holds cached
             free, _ = torch.cuda.mem_get_info(self.device)
-            mem_budget = int(free * 0.9)
+            cached = torch.cuda.memory_reserved(self.device) - torch.cuda.memory_allocated(self.device)
+            mem_budget = int((free + cached) * 0.9)
         self.mem_budget = int(mem_budget)
+        self.store_policy = store_policy
         self.steps_per_tick = int(steps_per_tick)
         self.engine = Engine(self.stream, max_units=max_units, max_tiles=max_tiles)
         self._queue: collections.deque = collections.deque()
@@ -107,8 +138,10 @@ class FWIServer:
         obs = obs.to(self.device, torch.float32).reshape(-1).contiguous()
         if obs.numel() != params.record_len:
             raise ValueError(f"record_obs has {obs.numel()} values, expected {params.record_len}")
-        ckpt = best_ckpt_interval(params) if ckpt_interval is None else int(ckpt_interval)
-        need = job_bytes(params, vel.shape[0], ckpt)
+        # without an explicit interval the store mode is chosen at admission;
+        # the checkpointed footprint (the smallest) must fit the budget
+        ckpt = None if ckpt_interval is None else int(ckpt_interval)
+        need = job_bytes(params, vel.shape[0], best_ckpt_interval(params) if ckpt is None else ckpt)
         if need > self.mem_budget:
             raise MemoryError(f"request needs {need / 1e9:.1f} GB, budget is {self.mem_budget / 1e9:.1f} GB")
         ev = torch.cuda.Event()
@@ -129,6 +162,8 @@ class FWIServer:
         if wait:
             self._thread.join()
         self.engine.close()
+        if wait:
+            torch.cuda.empty_cache()  # job workspaces released by the server thread
 
     def __enter__(self):
         return self
@@ -151,9 +186,32 @@ class FWIServer:
             if self._inflight and (self._mem_in_use + req.need > self.mem_budget
                                    or self._units_in_use + units > self.engine.max_units):
                 return
+            ckpt, need = req.ckpt, req.need
+            if ckpt is None:
+                full = job_bytes(req.params, req.vel.shape[0], 0)
+                if choose_store(full, need, self._mem_in_use, self.mem_budget, self.store_policy):
+                    ckpt, need = 0, full
+                else:
+                    ckpt = best_ckpt_interval(req.params)
+            try:
+                job = EngineJob(req.params, req.vel, req.obs, ckpt, req.run_fwi)
+            except torch.OutOfMemoryError:
+                # The budget is counted in bytes, but the caching allocator
+                # can be too fragmented to place the workspace: wait for
+                # running jobs to release theirs, or, with none running,
+                # return cached blocks and try once more.
+                if self._inflight:
+                    return
+                torch.cuda.empty_cache()
+                try:
+                    job = EngineJob(req.params, req.vel, req.obs, ckpt, req.run_fwi)
+                except torch.OutOfMemoryError as exc:
+                    self._queue.popleft()
+                    req.future.set_exception(MemoryError(f"no room for a {need / 1e9:.1f} GB job: {exc}"))
+                    continue
+            req.ckpt, req.need = ckpt, need
             self._queue.popleft()
             self.stream.wait_event(req.ready)
-            job = EngineJob(req.params, req.vel, req.obs, req.ckpt, req.run_fwi)
             self.engine.submit(job)
             req.job = job
             req.admitted = time.perf_counter()
@@ -170,6 +228,7 @@ class FWIServer:
             self._units_in_use -= req.vel.shape[0] * req.params.ns
             job = req.job
             res = FWIResult(job.misfit, job.grad, job.syn, req.submitted, req.admitted, time.perf_counter())
+            res.extra["ckpt_interval"] = req.ckpt
             job.workspace = None  # give the workspace back to the allocator
             self.completed.append(res)
             req.future.set_result(res)

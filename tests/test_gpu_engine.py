@@ -125,3 +125,22 @@ def test_server_concurrent_requests(three_jobs):
     for i, r in enumerate(results):
         syn, grad, mis = refs[i % 3]
         assert torch.equal(r.record_syn, syn) and torch.equal(r.grad, grad) and torch.equal(r.misfit, mis)
+
+
+@pytest.mark.parametrize("budget_gb,expect_full", [(None, True), (0.02, False)])
+def test_server_store_policy(three_jobs, budget_gb, expect_full):
+    """Requests without an interval: the server picks the full store when
+    it is small next to the budget, checkpoints otherwise; the results are
+    the same bits either way."""
+    from geopvi_amd.engine import best_ckpt_interval
+    from geopvi_amd.serving import FWIServer
+    refs = [solver.fwi_batch(p, v, o) for (p, v, o) in three_jobs]
+    torch.cuda.synchronize()
+    budget = None if budget_gb is None else int(budget_gb * (1 << 30))
+    with FWIServer(steps_per_tick=32, mem_budget=budget) as srv:
+        futs = [srv.submit(p, v, o) for (p, v, o) in three_jobs]
+        results = [f.result(timeout=120) for f in futs]
+    for (p, _, _), r, (syn, grad, mis) in zip(three_jobs, results, refs):
+        ck = r.extra["ckpt_interval"]
+        assert ck == (0 if expect_full else best_ckpt_interval(p)), (ck, p)
+        assert torch.equal(r.record_syn, syn) and torch.equal(r.grad, grad) and torch.equal(r.misfit, mis)
