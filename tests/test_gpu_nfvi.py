@@ -92,3 +92,48 @@ def test_inversion_device_matches_cpu_oracle_run(oracle_mod, tmp_path, float64_d
         assert diff <= lr * 1e-3, name
     # the parameters moved (the test is not vacuous)
     assert np.abs(q_cpu.flows[0].u.detach().numpy() - u0).max() > 0.5 * lr
+
+
+def test_twenty_iterations_mean_and_std_maps(oracle_mod, tmp_path, float64_default):
+    """ADVI for 20 iterations from one seed on the device and on the CPU
+    (oracle likelihood): the posterior mean and standard-deviation maps of
+    the two runs, each from 256 draws of one seeded generator, agree."""
+    p = synth.workload_params("tiny", nt=200)
+    vt = synth.gradient_anomaly_model(p.nz, p.nx, seed=4)
+    vs = synth.smooth(vt, 2.0)
+    obs = oracle_mod.forward(p, vt)
+    sigma = 0.05 * float(np.sqrt(np.mean(obs.astype(np.float64) ** 2)))
+    dim = p.ni
+    lo, up = np.full(dim, 1000.0), np.full(dim, 5000.0)
+    u0 = np.log(vs.reshape(-1) - lo) - np.log(up - vs.reshape(-1))
+    flows = [Linear(dim, kernel="diagonal", param=np.concatenate([u0, np.full(dim, 0.02)])),
+             Real2Constr(dim, lower=lo, upper=up)]
+    prior = Uniform(lower=lo, upper=up)
+    n_iter, nsample, lr = 20, 4, 0.01
+
+    q_cpu = FlowsBasedDistribution(copy.deepcopy(flows), generator=torch.Generator().manual_seed(21))
+    logp_cpu = lambda z: -OracleLikelihood.apply(z, oracle_mod, p, obs) / sigma ** 2 + prior.log_prob(z)  # noqa: E731
+    loss_cpu = VariationalInversion(q_cpu, logp_cpu).update(lr=lr, n_iter=n_iter, nsample=nsample)
+    post = Posterior(obs, fwi_config(p), vel_fixed=1500.0, sigma=sigma, log_prior=prior.log_prob,
+                     paramfile=str(tmp_path / "input_params.txt"))
+    q_dev = FlowsBasedDistribution(copy.deepcopy(flows), generator=torch.Generator().manual_seed(21)).cuda()
+    loss_dev = VariationalInversion(q_dev, post.log_prob).update(lr=lr, n_iter=n_iter, nsample=nsample)
+
+    def maps(q, device):
+        # the same 256 base draws through each run's flows
+        with torch.no_grad():
+            z = q.flows[0].forward(torch.from_numpy(np.random.default_rng(5).standard_normal((256, dim))).to(device))[0]
+            x = q.flows[1].forward(z)[0]
+        x = x.cpu().numpy()
+        return x.mean(axis=0), x.std(axis=0)
+
+    m_cpu, s_cpu = maps(q_cpu, "cpu")
+    m_dev, s_dev = maps(q_dev, "cuda")
+    rel_m = float(np.linalg.norm(m_dev - m_cpu) / np.linalg.norm(m_cpu))
+    rel_s = float(np.linalg.norm(s_dev - s_cpu) / np.linalg.norm(s_cpu))
+    print(f"20 iterations: loss rel {abs(loss_dev[-1] - loss_cpu[-1]) / abs(loss_cpu[-1]):.2e}, "
+          f"mean map rel {rel_m:.2e}, std map rel {rel_s:.2e}", flush=True)
+    np.testing.assert_allclose(loss_dev, loss_cpu, rtol=1e-4)
+    assert rel_m <= 1e-5 and rel_s <= 1e-5
+    # the run moved the maps away from the start
+    assert float(np.abs(m_cpu - vs.reshape(-1)).max()) > 1.0
