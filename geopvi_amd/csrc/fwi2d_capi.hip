@@ -740,7 +740,7 @@ int psm_plans(int ntz, int ntx, int batch, PsmPlans* out)
     return 0;
 }
 
-int psm_run(hipStream_t st, const Geom& g, const JobDesc& j, const BatchLayout& L, int nunit)
+int psm_run(hipStream_t st, const Geom& g, const Sched& sc, const JobDesc& j, const BatchLayout& L, int nunit)
 {
     if (nunit > 65535) return fail(GPVI_ERR_INVALID, "pseudo-spectral batches hold at most 65535 units");
     HIP_TRY(launch_psm_wavenumber(st, L.ktab, g.ntx, g.ntz, j.dx, j.dz));
@@ -751,8 +751,18 @@ int psm_run(hipStream_t st, const Geom& g, const JobDesc& j, const BatchLayout& 
     if (hipfftSetStream(pl.fwd, st) != HIPFFT_SUCCESS || hipfftSetStream(pl.inv, st) != HIPFFT_SUCCESS)
         return fail(GPVI_ERR_HIP, "hipfftSetStream failed");
     auto* spec = reinterpret_cast<hipfftDoubleComplex*>(L.fft);
+    // Every unit of a batch follows one schedule.  Within a pass the point
+    // kernel leaves each step's result in the transform buffer as the next
+    // step's input; the first step of a pass (forward, a segment's
+    // recompute, an adjoint stretch) packs its p1 from the field buffers.
+    UnitDesc sd;
+    std::memset(&sd, 0, sizeof sd);
+    sd.nt = g.nt; sd.tb = 1; sd.nseg = sc.nseg; sd.ckpt = sc.ckpt; sd.last_len = sc.last_len;
+    int prev_mode = -1;
     for (int s = 0; s < j.total; ++s) {
-        HIP_TRY(launch_psm_pack(st, L.units, nunit, s, L.fft, g.ntx, g.ntz));
+        const int mode = decode_step(sd, s + j.s_off).mode;
+        if (mode != prev_mode) HIP_TRY(launch_psm_pack(st, L.units, nunit, s, L.fft, g.ntx, g.ntz));
+        prev_mode = mode;
         if (hipfftExecD2Z(pl.fwd, L.fft, spec) != HIPFFT_SUCCESS)
             return fail(GPVI_ERR_HIP, "hipfftExecD2Z failed");
         HIP_TRY(launch_psm_scale(st, L.fft, L.ktab, g.ntx, g.ntz, nunit));
@@ -792,7 +802,7 @@ int gpvi_fwi2d_batch(const gpvi_fwi2d_params* p, int nsample, const float* vel_i
     JobDesc j = make_job(p, g, sc, nsample, L, record_obs, record_syn, run_fwi, phase, store);
     HIP_TRY(launch_build_units(st, j, L.units, L.tiles));
     if (g.lc == 0) {
-        if ((rc = psm_run(st, g, j, L, nsample * g.ns))) return rc;
+        if ((rc = psm_run(st, g, sc, j, L, nsample * g.ns))) return rc;
         if (phase != GPVI_PHASE_FORWARD)
             HIP_TRY(launch_grad_reduce(st, L.gpriv, L.grad_stride, grad, g.ns, g.ni, nsample));
         if (phase != GPVI_PHASE_ADJOINT && misfit)

@@ -69,7 +69,7 @@ int misfit_blocks(int64_t len);
 hipError_t launch_psm_wavenumber(hipStream_t st, float* k, int ntx, int ntz, float dx, float dz);
 hipError_t launch_psm_pack(hipStream_t st, const UnitDesc* units, int nunit, int s, double* buf, int ntx, int ntz);
 hipError_t launch_psm_scale(hipStream_t st, double* buf, const float* k, int ntx, int ntz, int nunit);
-hipError_t launch_psm_point(hipStream_t st, const UnitDesc* units, int nunit, int s, const double* buf, int ntx,
+hipError_t launch_psm_point(hipStream_t st, const UnitDesc* units, int nunit, int s, double* buf, int ntx,
                             int ntz);
 hipError_t launch_misfit(hipStream_t st, const float* syn, const float* obs, int64_t len, int nsample,
                          double* partial, double* misfit);

@@ -3,7 +3,9 @@
 // (pml = pml0, fdtd.c:75-85) and pstd_2d_calculate_p (fdtd.c:783-830) in
 // place of the stencil.  One time step of every unit of a batch:
 //
-//   pack    p1 -> double field                    (fdtd.c:791-799)
+//   pack    p1 -> double field                    (fdtd.c:791-799); only at
+//           the first step of a pass: within a pass the point kernel of
+//           the previous step leaves its result there
 //   FFT     batched 2-D forward transform         (hipFFT, D2Z)
 //   scale   multiply by -|k|^2                    (fdtd.c:805-813)
 //   IFFT    batched 2-D inverse (unnormalised)    (hipFFT, Z2D)
@@ -78,7 +80,7 @@ __global__ void psm_scale_kernel(double2* __restrict__ spec, const float* __rest
 }
 
 // Everything after the Laplacian, one thread per padded point of a unit.
-__global__ void psm_point_kernel(const UnitDesc* __restrict__ units, int s, const double* __restrict__ re_all,
+__global__ void psm_point_kernel(const UnitDesc* __restrict__ units, int s, double* __restrict__ re_all,
                                  int rp, int64_t ustride, int ntp)
 {
     const int un = blockIdx.y;
@@ -103,7 +105,8 @@ __global__ void psm_point_kernel(const UnitDesc* __restrict__ units, int s, cons
     const Rsrc rp0 = make_rsrc(f.p0 - margin);
     const Rsrc rv = make_rsrc(u.vel - margin);
     const float p1c = bload(rp1, vo, ro), p0c = bload(rp0, vo, ro), v = bload(rv, vo, ro);
-    const double re = re_all[un * ustride + (int64_t)iz * rp + ix];
+    double* const re_pt = re_all + un * ustride + (int64_t)iz * rp + ix;
+    const double re = *re_pt;
     const float vdt = v * kc.dt;
     float val = (float)((double)(2.f * p1c - p0c) + (double)(vdt * vdt) * re / (double)ntp);
 
@@ -146,6 +149,7 @@ __global__ void psm_point_kernel(const UnitDesc* __restrict__ units, int s, cons
     const bool inner = iz >= pml && iz < ntz - pml && ix >= pml && ix < ntx - pml;
     const unsigned gvo = inner ? (unsigned)(((iz - pml) * u.nx + (ix - pml)) * 4) : 0u;
     store_point(u, kc, c, val, p0c, p1c, v, ix, vo, ro, gvo, 0, make_rsrc(u.grad), rec_row, inner);
+    *re_pt = (double)val;  // the next step's transform input (this thread's own element)
 }
 
 hipError_t launch_psm_wavenumber(hipStream_t st, float* k, int ntx, int ntz, float dx, float dz)
@@ -177,7 +181,7 @@ hipError_t launch_psm_scale(hipStream_t st, double* buf, const float* k, int ntx
     return hipGetLastError();
 }
 
-hipError_t launch_psm_point(hipStream_t st, const UnitDesc* units, int nunit, int s, const double* buf, int ntx,
+hipError_t launch_psm_point(hipStream_t st, const UnitDesc* units, int nunit, int s, double* buf, int ntx,
                             int ntz)
 {
     const int ntp = ntx * ntz, rp = psm_rp(ntx);
