@@ -742,8 +742,9 @@ int psm_plans(int ntz, int ntx, int batch, PsmPlans* out)
 
 int psm_run(hipStream_t st, const Geom& g, const Sched& sc, const JobDesc& j, const BatchLayout& L, int nunit)
 {
-    if (nunit > 65535) return fail(GPVI_ERR_INVALID, "pseudo-spectral batches hold at most 65535 units");
     HIP_TRY(launch_psm_wavenumber(st, L.ktab, g.ntx, g.ntz, j.dx, j.dz));
+    // A plan holds one stream: the lock keeps two callers from sharing a
+    // cached plan at once (their batches run one after the other).
     std::lock_guard<std::mutex> lk(g_psm_mu);
     PsmPlans pl;
     int rc;
@@ -788,6 +789,8 @@ int gpvi_fwi2d_batch(const gpvi_fwi2d_params* p, int nsample, const float* vel_i
     const Tuning tn = tuning();
     Geom g;
     make_geom(p, tn.variant, tn.tb, &g);
+    if (g.lc == 0 && (int64_t)nsample * g.ns > 65535)  // one grid row of launches per unit
+        return fail(GPVI_ERR_INVALID, "pseudo-spectral batches hold at most 65535 (sample, shot) units");
     const bool model_only = phase == GPVI_PHASE_FORWARD && (flags & GPVI_FLAG_NO_STORE);
     Sched sc = make_sched(g, model_only ? 0 : ckpt_interval);
     BatchLayout L;

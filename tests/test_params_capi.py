@@ -139,3 +139,14 @@ def test_compat_entry_reports_bad_parameter_file(tmp_path):
     buf = (ctypes.c_float * 4)()
     lib.fwi_2D(str(tmp_path / "nope.txt").encode(), buf, buf, buf, buf, 1, 0)
     assert lib.gpvi_compat_last_status() == -4
+
+
+def test_pseudo_spectral_batch_unit_limit():
+    # the pseudo-spectral kernels launch one grid row per (sample, shot)
+    # unit: more than 65535 units is refused before any device work
+    p = synth.workload_params("tiny", laplace_slover=1)
+    c = p.to_c()
+    fake = ctypes.c_void_p(256)  # never dereferenced: the call returns first
+    rc = _lib.load().gpvi_fwi2d_batch(ctypes.byref(c), 40000, fake, fake, fake, fake, None, 1, 0, 0, 0, fake,
+                                      1 << 40, None)
+    assert rc == -1 and "65535" in _lib.last_error()
