@@ -113,13 +113,18 @@ def serve_stream(args, rate, nreq, seed, dev):
             futs.append(srv.submit(p, vel, obs))
         res = [f.result() for f in futs]
         t_end = max(r.finished for r in res)
+        arena_waits = srv.arena_waits
     lat = np.array([r.latency for r in res])
+    wait = np.array([r.admitted - r.submitted for r in res])
     span = t_end - t0
     n_full = sum(1 for r in res if r.extra.get("ckpt_interval") == 0)
     return {"requests": nreq, "rate_req_s": rate, "store": f"auto: {n_full} full store, {nreq - n_full} checkpointed",
+            "arena_waits": arena_waits,
             "p50_ms": round(float(np.percentile(lat, 50)) * 1e3, 1),
             "p99_ms": round(float(np.percentile(lat, 99)) * 1e3, 1),
-            "mean_ms": round(float(lat.mean()) * 1e3, 1), "makespan_s": round(span, 3),
+            "mean_ms": round(float(lat.mean()) * 1e3, 1),
+            "admission_wait_p50_ms": round(float(np.percentile(wait, 50)) * 1e3, 1),
+            "makespan_s": round(span, 3),
             "samples_per_s": round(total_samples / span, 3),
             "achieved_GBs": round(total_bytes / span / 1e9, 1)}
 
@@ -185,6 +190,16 @@ def vi_leg(args, dev):
         return out
     finally:
         torch.set_default_dtype(old)
+
+
+def _recorded_traffic():
+    """HBM bytes per launch of the c1 step kernel from the profiler run on
+    record (profiles/traffic_step_multi_c1.json), with its launch shape."""
+    path = os.path.join(REPO, "profiles", "traffic_step_multi_c1.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        return json.load(f)
 
 
 def psm_leg(args, dev, p_fd, samples, obs):
@@ -260,6 +275,16 @@ def main():
         autotune.apply(tuple(int(x) for x in args.tune.split(",")))
     elif not args.no_autotune:
         tune = autotune.autotune(p, args.nsample, cache_path=None)
+        # Shapes within 3 % of the fastest are the same speed to the
+        # autotuner's resolution (64 forward steps); prefer the one whose HBM
+        # counters are on record, so the roofline's traffic figure applies.
+        rec = _recorded_traffic()
+        if rec:
+            key = f"{rec['variant']},{rec['grid']},{rec['tb']}"
+            t_rec = tune["timings"].get(key)
+            if t_rec is not None and t_rec <= 1.03 * tune["us_per_step"]:
+                autotune.apply((rec["variant"], rec["grid"], rec["tb"]))
+                tune = {**tune, "applied": key, "note": "recorded-traffic shape within 3 % of the fastest"}
     variant, grid, graph_steps = _lib.get_tuning()
     tb = _lib.get_time_blocking()
     log(f"launch shape: variant {variant} grid {grid} steps/launch {tb}")
@@ -303,12 +328,9 @@ def main():
     avg_launch_s = gpu_ms * 1e-3 / (args.steps * launches)
     achieved = bytes_per_launch / avg_launch_s / 1e9
     traffic = None
-    tpath = os.path.join(REPO, "profiles", "traffic_step_multi_c1.json")
-    if os.path.exists(tpath):
-        with open(tpath) as f:
-            tj = json.load(f)
-        if tj.get("tb") == tb and tj.get("variant") == variant and tj.get("ckpt") == args.ckpt:
-            traffic = tj.get("hbm_bytes_per_launch")
+    tj = _recorded_traffic()
+    if tj and tj.get("tb") == tb and tj.get("variant") == variant and tj.get("ckpt") == args.ckpt:
+        traffic = tj.get("hbm_bytes_per_launch")
 
     out = {
         "metric": METRIC,

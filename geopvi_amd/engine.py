@@ -22,7 +22,11 @@ from .forward.fwi2d.params import FWIParams
 class EngineJob:
     """Device buffers of one submitted job; kept alive until it finishes."""
 
-    def __init__(self, params: FWIParams, vel: torch.Tensor, obs: torch.Tensor, ckpt: int, run_fwi: int):
+    def __init__(self, params: FWIParams, vel: torch.Tensor, obs: torch.Tensor, ckpt: int, run_fwi: int,
+                 workspace: Optional[torch.Tensor] = None):
+        """workspace: a uint8 device tensor of at least job_workspace_bytes
+        bytes, 256-byte aligned (e.g. a slice of a server's arena); allocated
+        here when None."""
         self.params = params
         self.vel = vel
         self.obs = obs
@@ -33,7 +37,13 @@ class EngineJob:
         self.syn = torch.empty(n, params.record_len, dtype=torch.float32, device=dev)
         self.grad = torch.zeros(n, params.ni, dtype=torch.float32, device=dev)
         self.misfit = torch.empty(n, dtype=torch.float64, device=dev)
-        self.workspace = torch.empty(job_workspace_bytes(params, n, ckpt), dtype=torch.uint8, device=dev)
+        need = job_workspace_bytes(params, n, ckpt)
+        if workspace is None:
+            workspace = torch.empty(need, dtype=torch.uint8, device=dev)
+        elif workspace.numel() < need or workspace.data_ptr() % 256:
+            raise ValueError(f"workspace: {workspace.numel()} bytes at {workspace.data_ptr():#x}, "
+                             f"need {need} bytes 256-byte aligned")
+        self.workspace = workspace
         self.id: Optional[int] = None
 
     @property

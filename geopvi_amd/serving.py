@@ -2,11 +2,19 @@
 
 Requests (a velocity batch of any grid size against observed data kept on
 the GPU) arrive from any thread.  The server thread admits queued requests
-in arrival order while their device footprint (checkpointed workspace +
-outputs) fits the memory budget and the engine has unit slots, advances all
-admitted jobs together -- every step launch covers every unit of every job
--- and resolves each request's future as soon as its outputs are final.
-Latency is measured from submit() to the moment the result is ready.
+in arrival order while their workspace (full wavefield store or
+checkpoints, see choose_store) can be placed in the server's arena and the
+engine has unit slots, advances all admitted jobs together -- every step
+launch covers every unit of every job -- and resolves each request's future
+as soon as its outputs are final.  Latency is measured from submit() to the
+moment the result is ready.
+
+The arena is one device allocation of the whole memory budget, made when
+the server starts; job workspaces are carved out of it by a host-side
+first-fit allocator, so admission is decided by the arena's free blocks and
+no device allocation or release (and none of the device-wide
+synchronisation that comes with them) happens while the server runs.  The
+small per-job outputs are ordinary tensors handed to the caller.
 
 This replaces the per-call process pool of the reference's likelihood
 (geopvi/forward/fwi2d/posterior.py:131-149): many inversions, each with
@@ -15,6 +23,7 @@ log_prob call.
 """
 from __future__ import annotations
 
+import bisect
 import collections
 import threading
 import time
@@ -25,7 +34,7 @@ from typing import Optional
 import numpy as np
 import torch
 
-from .engine import Engine, EngineJob, best_ckpt_interval, job_bytes
+from .engine import Engine, EngineJob, best_ckpt_interval, job_workspace_bytes
 from .forward.fwi2d import solver
 from .forward.fwi2d.params import FWIParams
 
@@ -45,23 +54,72 @@ class FWIResult:
         return self.finished - self.submitted
 
 
-def choose_store(full_bytes: int, ckpt_bytes: int, in_use: int, budget: int, policy: str = "auto",
+class Arena:
+    """First-fit allocator of 256-byte-aligned blocks over one device buffer
+    (host-side bookkeeping; the buffer is allocated once)."""
+
+    ALIGN = 256
+
+    def __init__(self, nbytes: int, device: torch.device):
+        self.size = int(nbytes) // self.ALIGN * self.ALIGN
+        self.buf = torch.empty(max(self.size, self.ALIGN), dtype=torch.uint8, device=device)
+        self._free = [(0, self.size)]   # sorted, coalesced (offset, size)
+
+    @classmethod
+    def _round(cls, n: int) -> int:
+        return (int(n) + cls.ALIGN - 1) // cls.ALIGN * cls.ALIGN
+
+    def largest(self) -> int:
+        return max((sz for _, sz in self._free), default=0)
+
+    def free_bytes(self) -> int:
+        return sum(sz for _, sz in self._free)
+
+    def alloc(self, nbytes: int) -> Optional[int]:
+        n = self._round(nbytes)
+        for i, (off, sz) in enumerate(self._free):
+            if sz >= n:
+                if sz == n:
+                    del self._free[i]
+                else:
+                    self._free[i] = (off + n, sz - n)
+                return off
+        return None
+
+    def release(self, off: int, nbytes: int) -> None:
+        n = self._round(nbytes)
+        i = bisect.bisect_left(self._free, (off, 0))
+        self._free.insert(i, (off, n))
+        # coalesce with the neighbours
+        if i + 1 < len(self._free) and off + n == self._free[i + 1][0]:
+            self._free[i] = (off, n + self._free[i + 1][1])
+            del self._free[i + 1]
+        if i > 0 and self._free[i - 1][0] + self._free[i - 1][1] == off:
+            self._free[i - 1] = (self._free[i - 1][0], self._free[i - 1][1] + self._free[i][1])
+            del self._free[i]
+
+    def view(self, off: int, nbytes: int) -> torch.Tensor:
+        return self.buf[off:off + self._round(nbytes)]
+
+
+def choose_store(full_bytes: int, ckpt_bytes: int, room: int, budget: int, policy: str = "auto",
                  full_share: int = 8) -> bool:
     """True: run the job with the full wavefield store; False: checkpointed.
 
-    The full store skips the recompute pass (two passes over the time axis
+    full_bytes / ckpt_bytes: the job's workspace in either mode; room: the
+    largest block that can be placed now; budget: the arena's size.  The
+    full store skips the recompute pass (two passes over the time axis
     instead of three) and gives identical results.  "auto" takes it when the
-    job's full-store footprint is at most 1/full_share of the budget and fits
-    in what is free now, so a few large jobs cannot crowd out concurrency;
-    "full" whenever it fits now; "checkpoint" never.  The caller has checked
-    that the checkpointed footprint fits."""
+    full-store workspace is at most 1/full_share of the budget and can be
+    placed now, so a few large jobs cannot crowd out concurrency; "full"
+    whenever it can be placed now; "checkpoint" never."""
     if policy not in ("auto", "full", "checkpoint"):
         raise ValueError(f"store policy {policy!r}: 'auto', 'full' or 'checkpoint'")
     if policy == "checkpoint":
         return False
     if full_bytes <= ckpt_bytes:  # checkpointing would save nothing
         return True
-    fits = in_use + full_bytes <= budget
+    fits = full_bytes <= room
     if policy == "full":
         return fits
     return fits and full_bytes * full_share <= budget
@@ -86,14 +144,13 @@ class _Request:
 class FWIServer:
     """Serve FWI-2D misfit+gradient requests with continuous batching.
 
-    mem_budget: bytes of device memory admitted jobs may hold (default: 90 %
-    of the free memory when the server starts, counting blocks torch's
-    allocator holds cached; they go back to the device when the server
-    closes).  steps_per_tick: time steps
-    advanced between admission/completion checks (rounded to whole graph
-    segments by the engine).  store_policy: how a request without an explicit
-    ckpt_interval keeps its forward wavefield (see choose_store); requests
-    are admitted by the footprint of the chosen mode.
+    mem_budget: bytes of the workspace arena (default: 90 % of the free
+    memory when the server starts, counting blocks torch's allocator holds
+    cached; the arena goes back when the server closes).  steps_per_tick:
+    time steps advanced between admission/completion checks (rounded to
+    whole graph segments by the engine).  store_policy: how a request
+    without an explicit ckpt_interval keeps its forward wavefield (see
+    choose_store).
     """
 
     def __init__(self, device: int = 0, mem_budget: Optional[int] = None, steps_per_tick: int = 64,
@@ -103,22 +160,22 @@ class FWIServer:
         torch.cuda.set_device(self.device)
         self.stream = torch.cuda.Stream(self.device)
         if mem_budget is None:
-            # free device memory plus what torch's allocator holds cached
             free, _ = torch.cuda.mem_get_info(self.device)
             cached = torch.cuda.memory_reserved(self.device) - torch.cuda.memory_allocated(self.device)
             mem_budget = int((free + cached) * 0.9)
-        self.mem_budget = int(mem_budget)
+        self.arena = Arena(mem_budget, self.device)
+        self.mem_budget = self.arena.size
         self.store_policy = store_policy
         self.steps_per_tick = int(steps_per_tick)
         self.engine = Engine(self.stream, max_units=max_units, max_tiles=max_tiles)
         self._queue: collections.deque = collections.deque()
         self._inflight: dict = {}
-        self._mem_in_use = 0
         self._units_in_use = 0
         self._cv = threading.Condition()
         self._stop = False
         self._error: Optional[BaseException] = None
         self.completed: list = []
+        self.arena_waits = 0  # admissions deferred until running jobs free arena space
         self._thread = threading.Thread(target=self._run, name="fwi-server", daemon=True)
         self._thread.start()
 
@@ -139,11 +196,12 @@ class FWIServer:
         if obs.numel() != params.record_len:
             raise ValueError(f"record_obs has {obs.numel()} values, expected {params.record_len}")
         # without an explicit interval the store mode is chosen at admission;
-        # the checkpointed footprint (the smallest) must fit the budget
+        # the checkpointed workspace (the smallest) must fit the arena
         ckpt = None if ckpt_interval is None else int(ckpt_interval)
-        need = job_bytes(params, vel.shape[0], best_ckpt_interval(params) if ckpt is None else ckpt)
-        if need > self.mem_budget:
-            raise MemoryError(f"request needs {need / 1e9:.1f} GB, budget is {self.mem_budget / 1e9:.1f} GB")
+        need = job_workspace_bytes(params, vel.shape[0], best_ckpt_interval(params) if ckpt is None else ckpt)
+        if need > self.arena.size:
+            raise MemoryError(f"request needs {need / 1e9:.1f} GB of workspace, the arena has "
+                              f"{self.arena.size / 1e9:.1f} GB")
         ev = torch.cuda.Event()
         ev.record()  # inputs written on the caller's stream are ready here
         fut: Future = Future()
@@ -163,7 +221,8 @@ class FWIServer:
             self._thread.join()
         self.engine.close()
         if wait:
-            torch.cuda.empty_cache()  # job workspaces released by the server thread
+            self.arena = None
+            torch.cuda.empty_cache()  # the arena's memory goes back to the device
 
     def __enter__(self):
         return self
@@ -176,47 +235,56 @@ class FWIServer:
         if lat.size == 0:
             return {"completed": 0}
         return {"completed": int(lat.size), "p50_s": float(np.percentile(lat, 50)),
-                "p99_s": float(np.percentile(lat, 99)), "mean_s": float(lat.mean())}
+                "p99_s": float(np.percentile(lat, 99)), "mean_s": float(lat.mean()),
+                "arena_waits": self.arena_waits}
 
     # -- server thread ---------------------------------------------------------
+    def _place(self, req):
+        """(ckpt, workspace bytes, arena offset) for the request, or None if
+        its workspace cannot be placed now."""
+        n = req.vel.shape[0]
+        if req.ckpt is not None:
+            need = job_workspace_bytes(req.params, n, req.ckpt)
+            off = self.arena.alloc(need)
+            return None if off is None else (req.ckpt, need, off)
+        full = job_workspace_bytes(req.params, n, 0)
+        if choose_store(full, req.need, self.arena.largest(), self.arena.size, self.store_policy):
+            off = self.arena.alloc(full)
+            if off is not None:
+                return 0, full, off
+        off = self.arena.alloc(req.need)
+        return None if off is None else (best_ckpt_interval(req.params), req.need, off)
+
     def _admit(self):
         while self._queue:
             req = self._queue[0]
             units = req.vel.shape[0] * req.params.ns
-            if self._inflight and (self._mem_in_use + req.need > self.mem_budget
-                                   or self._units_in_use + units > self.engine.max_units):
+            if self._inflight and self._units_in_use + units > self.engine.max_units:
                 return
-            ckpt, need = req.ckpt, req.need
-            if ckpt is None:
-                full = job_bytes(req.params, req.vel.shape[0], 0)
-                if choose_store(full, need, self._mem_in_use, self.mem_budget, self.store_policy):
-                    ckpt, need = 0, full
-                else:
-                    ckpt = best_ckpt_interval(req.params)
+            placed = self._place(req)
+            if placed is None:
+                # every job's checkpointed workspace fits the empty arena
+                # (submit() checks), so running jobs will make room
+                self.arena_waits += 1
+                return
+            ckpt, need, off = placed
             try:
-                job = EngineJob(req.params, req.vel, req.obs, ckpt, req.run_fwi)
-            except torch.OutOfMemoryError:
-                # The budget is counted in bytes, but the caching allocator
-                # can be too fragmented to place the workspace: wait for
-                # running jobs to release theirs, or, with none running,
-                # return cached blocks and try once more.
+                job = EngineJob(req.params, req.vel, req.obs, ckpt, req.run_fwi,
+                                workspace=self.arena.view(off, need))
+            except torch.OutOfMemoryError as exc:  # the outputs (outside the arena)
+                self.arena.release(off, need)
                 if self._inflight:
                     return
-                torch.cuda.empty_cache()
-                try:
-                    job = EngineJob(req.params, req.vel, req.obs, ckpt, req.run_fwi)
-                except torch.OutOfMemoryError as exc:
-                    self._queue.popleft()
-                    req.future.set_exception(MemoryError(f"no room for a {need / 1e9:.1f} GB job: {exc}"))
-                    continue
-            req.ckpt, req.need = ckpt, need
+                self._queue.popleft()
+                req.future.set_exception(MemoryError(f"no room for the outputs: {exc}"))
+                continue
             self._queue.popleft()
+            req.ckpt, req.need, req.extra["arena_offset"] = ckpt, need, off
             self.stream.wait_event(req.ready)
             self.engine.submit(job)
             req.job = job
             req.admitted = time.perf_counter()
             self._inflight[job.id] = req
-            self._mem_in_use += req.need
             self._units_in_use += units
 
     def _reap(self):
@@ -224,12 +292,14 @@ class FWIServer:
         for jid in done:
             req = self._inflight.pop(jid)
             self.engine.release(jid)
-            self._mem_in_use -= req.need
             self._units_in_use -= req.vel.shape[0] * req.params.ns
             job = req.job
             res = FWIResult(job.misfit, job.grad, job.syn, req.submitted, req.admitted, time.perf_counter())
             res.extra["ckpt_interval"] = req.ckpt
-            job.workspace = None  # give the workspace back to the allocator
+            job.workspace = None
+            # the job's last kernels are complete (status 1), so its arena
+            # block can be handed to the next job
+            self.arena.release(req.extra["arena_offset"], req.need)
             self.completed.append(res)
             req.future.set_result(res)
 
