@@ -188,7 +188,8 @@ __global__ __launch_bounds__(kMisfitThreads) void misfit_partial_kernel(
     const float* __restrict__ syn, const float* __restrict__ obs, int64_t len, int nblk,
     double* __restrict__ partial)
 {
-    __shared__ double red[kMisfitThreads];
+    constexpr int kWaves = kMisfitThreads / 64;
+    __shared__ double wave_sum[kWaves];
     const int m = blockIdx.y, b = blockIdx.x;
     const int64_t chunk = (len + nblk - 1) / nblk;
     const int64_t lo = (int64_t)b * chunk;
@@ -199,13 +200,18 @@ __global__ __launch_bounds__(kMisfitThreads) void misfit_partial_kernel(
         const double d = (double)(sm[i] - obs[i]);
         acc += d * d;
     }
-    red[threadIdx.x] = acc;
+    // Wavefront butterfly (lane shuffles, a fixed order: deterministic),
+    // then the block's waves in order.
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 64);
+    if ((threadIdx.x & 63) == 0) wave_sum[threadIdx.x >> 6] = acc;
     __syncthreads();
-    for (int w = kMisfitThreads / 2; w > 0; w >>= 1) {
-        if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
-        __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = 0.0;
+#pragma unroll
+        for (int w = 0; w < kWaves; ++w) t += wave_sum[w];
+        partial[(int64_t)m * nblk + b] = t;
     }
-    if (threadIdx.x == 0) partial[(int64_t)m * nblk + b] = red[0];
 }
 
 __global__ void misfit_final_kernel(const double* __restrict__ partial, int nblk, int nsample,
