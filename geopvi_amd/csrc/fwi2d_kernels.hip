@@ -129,10 +129,17 @@ __global__ void build_units_kernel(JobDesc j, UnitDesc* __restrict__ units, Tile
     const int nunit = j.nsample * j.ns;
     const int tpu = 2 * j.band_tz + j.tiles_x * j.tiles_z;
     const int64_t ntile = (int64_t)nunit * tpu;
+    // Item order: sample, item position, shot.  The shots of one sample read
+    // the same velocity tile at the same item position; next to each other
+    // in the list, they run at about the same time on the same XCD, so the
+    // tile comes from HBM once and from L2 for the other shots.
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < ntile;
          i += (int64_t)gridDim.x * blockDim.x) {
-        const int un = (int)(i / tpu);
-        tiles[j.tile0 + i] = TileRef{j.unit0 + un, (int)(i - (int64_t)un * tpu)};
+        const int64_t mt = i / j.ns;                       // sample * tpu + item position
+        const int sh = (int)(i - mt * j.ns);
+        const int m = (int)(mt / tpu);
+        const int t = (int)(mt - (int64_t)m * tpu);
+        tiles[j.tile0 + i] = TileRef{j.unit0 + m * j.ns + sh, t};
     }
     const int un = blockIdx.x * blockDim.x + threadIdx.x;
     if (un >= nunit) return;
