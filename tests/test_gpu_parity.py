@@ -135,3 +135,28 @@ def test_side_band_items(oracle_mod, lc, k, ckpt):
     assert rel_l2(grad_p, grad_o) <= GRAD_RTOL
     assert rel_l2(grad_b[0], grad_o) <= GRAD_RTOL
     assert rel_err(mis_b[0], oracle_mod.misfit_f32(syn_o, obs)) <= MISFIT_RTOL
+
+
+@pytest.mark.parametrize("ckpt", [0, 40])
+def test_sources_and_receivers_inside_the_absorbing_bands(oracle_mod, ckpt):
+    """The reference places sources and receivers anywhere in the padded
+    grid (fdtd2D_modelling.c:125-152, no bounds checks): here the first
+    source sits in the left band's copy columns (ix < Lc), the last one in
+    the right band, all at a depth inside the top band, and the receivers
+    span every padded column at a depth inside the bottom band (above its
+    last Lc rows, which the reference never updates)."""
+    p, vt, vs = case("small", seed=17, nx=100, nz=30, nt=520, ns=3)
+    pml = p.pml0 + p.Lc + 1
+    ntx, ntz = p.nx + 2 * pml, p.nz + 2 * pml
+    s_first, s_last = 2, ntx - 3
+    p = p.replace(ns0=s_first - pml, ds=(s_last - s_first) // (p.ns - 1), depths=6 - pml,
+                  nr0=-pml, nr=ntx, dr=1, depthr=ntz - 7 - pml)
+    obs = oracle_mod.forward(p, vt)
+    syn_o, grad_o = oracle_mod.fwi(p, vs, obs)
+    syn_p, grad_p = (host(t) for t in solver.fwi_plain(p, dev(vs), dev(obs)))
+    syn_b, grad_b, mis_b = (host(t) for t in solver.fwi_batch(p, dev(vs).view(1, -1), dev(obs), ckpt_interval=ckpt))
+    _report(f"in-band geometry ckpt={ckpt}", syn_b[0], syn_o, grad_b[0], grad_o)
+    assert np.abs(syn_o).max() > 0
+    assert np.array_equal(syn_p, syn_o) and np.array_equal(syn_b[0], syn_o)
+    assert rel_l2(grad_p, grad_o) <= GRAD_RTOL and rel_l2(grad_b[0], grad_o) <= GRAD_RTOL
+    assert rel_err(mis_b[0], oracle_mod.misfit_f32(syn_o, obs)) <= MISFIT_RTOL
