@@ -237,7 +237,7 @@ def psm_leg(args, dev, p_fd, samples, obs):
 def main():
     args = parse()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is design-only "
+        print(json.dumps({"error": "the solver runs on one GPU; multi-GPU sharding is design-only "
                                    "(whole (sample, shot) units per GPU, one gradient reduce per sample)",
                           "n_gpus": args.gpus}))
         sys.exit(2)
