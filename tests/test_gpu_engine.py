@@ -144,3 +144,20 @@ def test_server_store_policy(three_jobs, budget_gb, expect_full):
         ck = r.extra["ckpt_interval"]
         assert ck == (0 if expect_full else best_ckpt_interval(p)), (ck, p)
         assert torch.equal(r.record_syn, syn) and torch.equal(r.grad, grad) and torch.equal(r.misfit, mis)
+
+
+def test_two_steps_per_launch_engine_matches_batches(three_jobs):
+    """Engine created under time blocking 2 (two steps per launch): a
+    staggered mix of the three jobs gives the standalone batches' bits."""
+    from geopvi_amd import _lib
+    refs = [[t.clone() for t in solver.fwi_batch(p, v, o, ckpt_interval=ck)]
+            for (p, v, o), ck in zip(three_jobs, (0, 64, 0))]
+    torch.cuda.synchronize()
+    saved = _lib.get_time_blocking()
+    _lib.set_time_blocking(2)
+    try:
+        mix = run_engine([(p, v, o, ck) for (p, v, o), ck in zip(three_jobs, (0, 64, 0))], stagger=96)
+    finally:
+        _lib.set_time_blocking(saved)
+    for j, (syn, grad, mis) in zip(mix, refs):
+        assert torch.equal(j.syn, syn) and torch.equal(j.grad, grad) and torch.equal(j.misfit, mis)
