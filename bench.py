@@ -13,7 +13,9 @@ Printed (one JSON line): batched samples/s (``value``), the plain
 one-sample-per-launch GPU solver's samples/s on the same workload and the
 ratio, the gradient agreement of the two, the roofline of the dominant
 kernel (the multi-unit step), a served request stream (continuous
-batching; p50/p99 latency) and the CPU oracle timed on this host.
+batching; p50/p99 latency), ADVI/PSVI iterations on the device, the
+pseudo-spectral path, the CPU oracle timed on this host, and the
+CPU-runnable c0 case on CPU (one worker and a per-sample fan-out) and GPU.
 """
 from __future__ import annotations
 
@@ -190,6 +192,86 @@ def vi_leg(args, dev):
         return out
     finally:
         torch.set_default_dtype(old)
+
+
+def _cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def c0_baselines(args, ncores, dev):
+    """The CPU-runnable case (c0: 120x200, 4 shots, 1000 steps, one sample):
+    the C oracle with all host cores (median of 3 after a warm-up); the
+    reference's per-sample fan-out (posterior.py:143-144 runs one worker per
+    sample, each with its own OpenMP threads) emulated as P concurrent
+    workers x T threads, P*T = cores (ctypes releases the GIL, and the
+    OpenMP thread count is per calling thread); and the plain and batched
+    GPU solvers on the same case."""
+    import threading
+    import torch
+    import oracle
+    from geopvi_amd import synth
+    from geopvi_amd.forward.fwi2d import solver
+
+    p0 = synth.workload_params("c0")
+    vt = synth.true_model("c0", seed=args.seed)
+    vs = synth.smooth(vt, 8.0).astype(np.float32)
+    oracle.set_threads(ncores)
+    obs = oracle.forward(p0, vt)
+    oracle.fwi(p0, vs, obs)
+    ts = []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        oracle.fwi(p0, vs, obs)
+        ts.append(time.perf_counter() - t0)
+    med = float(np.median(ts))
+    nproc = 4 if ncores >= 8 else 2
+    nthr = max(1, ncores // nproc)
+    batch = synth.gaussian_samples(vs, 50.0, nproc, seed=args.seed + 500).astype(np.float32)
+
+    def worker(i):
+        oracle.set_threads(nthr)
+        oracle.fwi(p0, batch[i], obs)
+
+    t0 = time.perf_counter()
+    th = [threading.Thread(target=worker, args=(i,)) for i in range(nproc)]
+    for t in th:
+        t.start()
+    for t in th:
+        t.join()
+    pool_s = time.perf_counter() - t0
+    oracle.set_threads(ncores)
+    log(f"c0 cpu: {med:.3f} s/sample on {ncores} threads; fan-out {nproc}x{nthr}: {pool_s:.3f} s")
+
+    vd = torch.from_numpy(vs.reshape(-1)).to(dev)
+    od = torch.from_numpy(obs).to(dev)
+    solver.fwi_plain(p0, vd, od)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(3):
+        solver.fwi_plain(p0, vd, od)
+    torch.cuda.synchronize()
+    plain = (time.perf_counter() - t0) / 3
+    vb = torch.from_numpy(synth.gaussian_samples(vs, 50.0, 16, seed=args.seed + 600).astype(np.float32)).to(dev)
+    solver.fwi_batch(p0, vb, od)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    solver.fwi_batch(p0, vb, od)
+    torch.cuda.synchronize()
+    batched = time.perf_counter() - t0
+    return {"workload": "c0: 120x200 grid, 4 shots, 100 receivers, 1000 steps",
+            "cpu_model": _cpu_model(), "cores": ncores,
+            "cpu_oracle_s_per_sample_median3": round(med, 4),
+            "cpu_fanout": {"workers": nproc, "threads_each": nthr, "samples": nproc,
+                           "samples_per_s": round(nproc / pool_s, 3)},
+            "gpu_plain_samples_per_s": round(1.0 / plain, 2),
+            "gpu_batched_16_samples_per_s": round(16 / batched, 2)}
 
 
 def _recorded_traffic():
@@ -413,6 +495,7 @@ def main():
                                "kind": "port",
                                "sample": f"{shots} of {p.ns} shots of one c1 sample through the C oracle "
                                          f"(OpenMP, reference loop order): {tc:.2f} s, scaled to a sample"}
+        out["c0_baselines"] = c0_baselines(args, ncores, dev)
     print(json.dumps(out), flush=True)
 
 
