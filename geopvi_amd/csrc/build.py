@@ -4,7 +4,11 @@
 
 Flags: -ffp-contract=off keeps every float operation separately rounded, as
 the reference's x86-64 build is; the packed-FP32 target feature is disabled
-(hipcc -O3 would otherwise form v_pk_* FP32 instructions).
+(hipcc -O3 would otherwise form v_pk_* FP32 instructions).  The step
+kernels are scheduled for instruction-level parallelism instead of the
+default occupancy target (their occupancy is fixed by an attribute): the
+full forward + adjoint step on the 16-sample 200x400 batch takes 141.6 us
+instead of 147.2 us per launch.  Scheduling changes no arithmetic.
 """
 from __future__ import annotations
 
@@ -18,6 +22,7 @@ REPO = os.path.dirname(PKG)
 LIB = os.path.join(PKG, "libgeopvi_fwi2d.so")
 SOURCES = ["fwi2d_kernels.hip", "fwi2d_psm.hip", "fwi2d_capi.hip"]
 HEADERS = ["fwi2d_device.h", "fwi2d_step.h", "fwi2d_kernels.h"]
+SOURCE_FLAGS = {"fwi2d_kernels.hip": ["-mllvm", "-amdgpu-sched-strategy=max-ilp"]}
 ARCH = os.environ.get("GPVI_OFFLOAD_ARCH", "gfx950")
 
 HIPCC_FLAGS = [
@@ -47,7 +52,7 @@ def build(verbose: bool = False) -> str:
     inc = ["-I", os.path.join(REPO, "include"), "-I", HERE]
     for src in SOURCES:
         obj = os.path.join(build_dir, src.replace(".hip", ".o"))
-        cmd = [_hipcc(), *HIPCC_FLAGS, *inc, "-c", os.path.join(HERE, src), "-o", obj]
+        cmd = [_hipcc(), *HIPCC_FLAGS, *SOURCE_FLAGS.get(src, []), *inc, "-c", os.path.join(HERE, src), "-o", obj]
         _run(cmd, verbose)
         objs.append(obj)
     tmp = LIB + ".tmp"
