@@ -23,10 +23,13 @@ class EngineJob:
     """Device buffers of one submitted job; kept alive until it finishes."""
 
     def __init__(self, params: FWIParams, vel: torch.Tensor, obs: torch.Tensor, ckpt: int, run_fwi: int,
-                 workspace: Optional[torch.Tensor] = None):
+                 workspace: Optional[torch.Tensor] = None, outputs=None):
         """workspace: a uint8 device tensor of at least job_workspace_bytes
         bytes, 256-byte aligned (e.g. a slice of a server's arena); allocated
-        here when None."""
+        here when None.  outputs: (record_syn (n, ns*nt*nr) float32,
+        grad (n, nz*nx) float32, zeroed -- the solver adds to it --, misfit
+        (n,) float64), contiguous device tensors (e.g. row slices of a larger
+        request's outputs); allocated here when None."""
         self.params = params
         self.vel = vel
         self.obs = obs
@@ -34,9 +37,16 @@ class EngineJob:
         self.run_fwi = run_fwi
         n = vel.shape[0]
         dev = vel.device
-        self.syn = torch.empty(n, params.record_len, dtype=torch.float32, device=dev)
-        self.grad = torch.zeros(n, params.ni, dtype=torch.float32, device=dev)
-        self.misfit = torch.empty(n, dtype=torch.float64, device=dev)
+        if outputs is None:
+            self.syn = torch.empty(n, params.record_len, dtype=torch.float32, device=dev)
+            self.grad = torch.zeros(n, params.ni, dtype=torch.float32, device=dev)
+            self.misfit = torch.empty(n, dtype=torch.float64, device=dev)
+        else:
+            self.syn, self.grad, self.misfit = outputs
+            for t, shape, dt in ((self.syn, (n, params.record_len), torch.float32),
+                                 (self.grad, (n, params.ni), torch.float32), (self.misfit, (n,), torch.float64)):
+                if tuple(t.shape) != shape or t.dtype != dt or not t.is_contiguous() or t.device != dev:
+                    raise ValueError(f"output {tuple(t.shape)} {t.dtype}: expected {shape} {dt}, contiguous on {dev}")
         need = job_workspace_bytes(params, n, ckpt)
         if workspace is None:
             workspace = torch.empty(need, dtype=torch.uint8, device=dev)

@@ -29,7 +29,7 @@ import threading
 import time
 from concurrent.futures import Future
 from dataclasses import dataclass, field
-from typing import Optional
+from typing import Optional, Tuple
 
 import numpy as np
 import torch
@@ -126,6 +126,22 @@ def choose_store(full_bytes: int, ckpt_bytes: int, room: int, budget: int, polic
 
 
 @dataclass
+class _Split:
+    """A request whose samples run as several engine jobs (chunks of samples
+    whose workspace fits the arena); they write into row slices of the
+    request's outputs (allocated when the first chunk is admitted), and the
+    request resolves when the last chunk finishes."""
+    nsample: int
+    remaining: int
+    submitted: float
+    admitted: float = 0.0
+    ckpts: list = field(default_factory=list)
+    syn: Optional[torch.Tensor] = None
+    grad: Optional[torch.Tensor] = None
+    misfit: Optional[torch.Tensor] = None
+
+
+@dataclass
 class _Request:
     params: FWIParams
     vel: torch.Tensor
@@ -139,14 +155,19 @@ class _Request:
     admitted: float = 0.0
     job: Optional[EngineJob] = None
     extra: dict = field(default_factory=dict)
+    split: Optional[_Split] = None   # set on the chunks of a split request
+    rows: Tuple[int, int] = (0, 0)   # this chunk's samples in the request
 
 
 class FWIServer:
     """Serve FWI-2D misfit+gradient requests with continuous batching.
 
-    mem_budget: bytes of the workspace arena (default: 90 % of the free
+    mem_budget: bytes of the workspace arena (default: 80 % of the free
     memory when the server starts, counting blocks torch's allocator holds
-    cached; the arena goes back when the server closes).  steps_per_tick:
+    cached; the arena goes back when the server closes).  The rest holds
+    the requests' inputs and outputs.  A request whose checkpointed
+    workspace exceeds half the arena is split into chunks of samples that
+    fit, run as separate jobs into row slices of its outputs.  steps_per_tick:
     time steps advanced between admission/completion checks (rounded to
     whole graph segments by the engine).  store_policy: how a request
     without an explicit ckpt_interval keeps its forward wavefield (see
@@ -162,7 +183,7 @@ class FWIServer:
         if mem_budget is None:
             free, _ = torch.cuda.mem_get_info(self.device)
             cached = torch.cuda.memory_reserved(self.device) - torch.cuda.memory_allocated(self.device)
-            mem_budget = int((free + cached) * 0.9)
+            mem_budget = int((free + cached) * 0.8)
         self.arena = Arena(mem_budget, self.device)
         self.mem_budget = self.arena.size
         self.store_policy = store_policy
@@ -198,20 +219,51 @@ class FWIServer:
         # without an explicit interval the store mode is chosen at admission;
         # the checkpointed workspace (the smallest) must fit the arena
         ckpt = None if ckpt_interval is None else int(ckpt_interval)
-        need = job_workspace_bytes(params, vel.shape[0], best_ckpt_interval(params) if ckpt is None else ckpt)
-        if need > self.arena.size:
-            raise MemoryError(f"request needs {need / 1e9:.1f} GB of workspace, the arena has "
-                              f"{self.arena.size / 1e9:.1f} GB")
-        ev = torch.cuda.Event()
-        ev.record()  # inputs written on the caller's stream are ready here
+        ck_plan = best_ckpt_interval(params) if ckpt is None else ckpt
+        n = vel.shape[0]
+        need = job_workspace_bytes(params, n, ck_plan)
+        chunk = n
+        if need > self.arena.size // 2:
+            # Too large for one job next to others: split the samples into
+            # chunks whose workspace is at most half the arena (two chunks
+            # can then run side by side), each written into its rows.
+            chunk = self._chunk_samples(params, n, ck_plan, self.arena.size // 2)
+            if chunk == 0:
+                raise MemoryError(f"one sample needs {job_workspace_bytes(params, 1, ck_plan) / 1e9:.1f} GB of "
+                                  f"workspace; the arena has {self.arena.size / 1e9:.1f} GB")
         fut: Future = Future()
-        req = _Request(params, vel, obs, int(run_fwi), ckpt, need, ev, fut, t_sub)
+        reqs = []
+        if chunk >= n:
+            ev = torch.cuda.Event()
+            ev.record()  # inputs written on the caller's stream are ready here
+            reqs.append(_Request(params, vel, obs, int(run_fwi), ckpt, need, ev, fut, t_sub))
+        else:
+            split = _Split(n, (n + chunk - 1) // chunk, t_sub)
+            ev = torch.cuda.Event()
+            ev.record()  # inputs written on the caller's stream are ready here
+            for lo in range(0, n, chunk):
+                hi = min(n, lo + chunk)
+                reqs.append(_Request(params, vel[lo:hi], obs, int(run_fwi), ckpt,
+                                     job_workspace_bytes(params, hi - lo, ck_plan), ev, fut, t_sub,
+                                     split=split, rows=(lo, hi)))
         with self._cv:
             if self._error is not None:
                 raise RuntimeError("server stopped after an error") from self._error
-            self._queue.append(req)
+            self._queue.extend(reqs)
             self._cv.notify()
         return fut
+
+    @staticmethod
+    def _chunk_samples(params: FWIParams, n: int, ckpt: int, limit: int) -> int:
+        """Most samples (<= n) whose checkpointed workspace is <= limit."""
+        lo, hi = 0, n
+        while lo < hi:
+            mid = (lo + hi + 1) // 2
+            if job_workspace_bytes(params, mid, ckpt) <= limit:
+                lo = mid
+            else:
+                hi = mid - 1
+        return lo
 
     def close(self, wait: bool = True):
         with self._cv:
@@ -268,15 +320,30 @@ class FWIServer:
                 self.arena_waits += 1
                 return
             ckpt, need, off = placed
+            outputs = None
+            sp = req.split
             try:
+                if sp is not None:
+                    if sp.syn is None:  # first chunk: the request's outputs, on the server's stream
+                        n, p = sp.nsample, req.params
+                        sp.syn = torch.empty(n, p.record_len, dtype=torch.float32, device=self.device)
+                        sp.grad = torch.zeros(n, p.ni, dtype=torch.float32, device=self.device)
+                        sp.misfit = torch.empty(n, dtype=torch.float64, device=self.device)
+                    lo, hi = req.rows
+                    outputs = (sp.syn[lo:hi], sp.grad[lo:hi], sp.misfit[lo:hi])
                 job = EngineJob(req.params, req.vel, req.obs, ckpt, req.run_fwi,
-                                workspace=self.arena.view(off, need))
+                                workspace=self.arena.view(off, need), outputs=outputs)
             except torch.OutOfMemoryError as exc:  # the outputs (outside the arena)
                 self.arena.release(off, need)
                 if self._inflight:
                     return
+                # nothing running will free memory: fail the request (all its
+                # queued chunks, if it was split)
                 self._queue.popleft()
-                req.future.set_exception(MemoryError(f"no room for the outputs: {exc}"))
+                if sp is not None:
+                    self._queue = collections.deque(r for r in self._queue if r.split is not sp)
+                if not req.future.done():
+                    req.future.set_exception(MemoryError(f"no room for the outputs: {exc}"))
                 continue
             self._queue.popleft()
             req.ckpt, req.need, req.extra["arena_offset"] = ckpt, need, off
@@ -284,6 +351,8 @@ class FWIServer:
             self.engine.submit(job)
             req.job = job
             req.admitted = time.perf_counter()
+            if req.split is not None and not req.split.admitted:
+                req.split.admitted = req.admitted
             self._inflight[job.id] = req
             self._units_in_use += units
 
@@ -294,14 +363,25 @@ class FWIServer:
             self.engine.release(jid)
             self._units_in_use -= req.vel.shape[0] * req.params.ns
             job = req.job
-            res = FWIResult(job.misfit, job.grad, job.syn, req.submitted, req.admitted, time.perf_counter())
-            res.extra["ckpt_interval"] = req.ckpt
             job.workspace = None
             # the job's last kernels are complete (status 1), so its arena
             # block can be handed to the next job
             self.arena.release(req.extra["arena_offset"], req.need)
+            sp = req.split
+            if sp is None:
+                res = FWIResult(job.misfit, job.grad, job.syn, req.submitted, req.admitted, time.perf_counter())
+                res.extra["ckpt_interval"] = req.ckpt
+            else:
+                sp.remaining -= 1
+                sp.ckpts.append(req.ckpt)
+                if sp.remaining:
+                    continue
+                res = FWIResult(sp.misfit, sp.grad, sp.syn, sp.submitted, sp.admitted, time.perf_counter())
+                res.extra["ckpt_interval"] = sp.ckpts[0] if len(set(sp.ckpts)) == 1 else list(sp.ckpts)
+                res.extra["chunks"] = len(sp.ckpts)
             self.completed.append(res)
-            req.future.set_result(res)
+            if not req.future.done():  # a failed split's remaining chunks resolve nothing
+                req.future.set_result(res)
 
     def _run(self):
         torch.cuda.set_device(self.device)

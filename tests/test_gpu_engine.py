@@ -161,3 +161,25 @@ def test_two_steps_per_launch_engine_matches_batches(three_jobs):
         _lib.set_time_blocking(saved)
     for j, (syn, grad, mis) in zip(mix, refs):
         assert torch.equal(j.syn, syn) and torch.equal(j.grad, grad) and torch.equal(j.misfit, mis)
+
+
+def test_server_splits_requests_larger_than_half_the_arena(three_jobs):
+    """An arena that holds one sample's checkpointed workspace per half:
+    a 3-sample request runs as 3 one-sample jobs writing into row slices of
+    its outputs, with the standalone batch's bits; a request one sample of
+    which does not fit is refused at submit()."""
+    from geopvi_amd.engine import best_ckpt_interval, job_workspace_bytes
+    from geopvi_amd.serving import FWIServer
+    p, v, o = three_jobs[0]
+    assert v.shape[0] == 3
+    ck = best_ckpt_interval(p)
+    w1, w2 = job_workspace_bytes(p, 1, ck), job_workspace_bytes(p, 2, ck)
+    syn, grad, mis = (t.clone() for t in solver.fwi_batch(p, v, o))
+    torch.cuda.synchronize()
+    with FWIServer(steps_per_tick=32, mem_budget=w1 + w2) as srv:
+        res = srv.submit(p, v, o).result(timeout=120)
+        with pytest.raises(MemoryError):
+            srv.submit(p.replace(nx=p.nx * 4, nz=p.nz * 4), torch.zeros(1, p.ni * 16, device="cuda"),
+                       torch.zeros(p.record_len, device="cuda"))
+    assert res.extra["chunks"] == 3
+    assert torch.equal(res.record_syn, syn) and torch.equal(res.grad, grad) and torch.equal(res.misfit, mis)
