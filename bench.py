@@ -58,6 +58,9 @@ def parse():
     ap.add_argument("--serve-rate", type=float, default=50.0, help="Poisson arrival rate (requests/s)")
     ap.add_argument("--serve-low-rate", type=float, default=10.0, help="sub-saturation run rate (0 = skip)")
     ap.add_argument("--serve-low-requests", type=int, default=60)
+    ap.add_argument("--serve-c3", type=int, default=0,
+                    help="also stream this many requests of the full C3 mix (grids up to 400x800, up to 64 "
+                         "samples x 40 shots x 3000 steps; minutes of device time)")
     ap.add_argument("--cpu-shots", type=int, default=20)
     ap.add_argument("--no-vi", action="store_true", help="skip the end-to-end inversion leg")
     ap.add_argument("--vi-iterations", type=int, default=2)
@@ -78,14 +81,16 @@ def algorithmic_bytes_per_unit(p) -> int:
     return p.nt * p.ntp * 32 + nsave * p.ni * 16 + p.nt * p.nr * 12
 
 
-def serve_stream(args, rate, nreq, seed, dev):
+def serve_stream(args, rate, nreq, seed, dev, mix="bench"):
     """A Poisson stream of misfit+gradient requests through the server.
 
-    Requests follow the serving generator's "bench" mix (grids 100x200 and
-    200x400, 4-16 samples, 5-10 shots, 500-1000 steps; velocity samples
-    around a smoothed model of the c0 generator at that size, observed
-    data modelled from the unsmoothed model, both prepared before the
-    stream starts).  Latency runs from submit() to the result being ready."""
+    Requests follow one of the serving generator's mixes: "bench" (grids
+    100x200 and 200x400, 4-16 samples, 5-10 shots, 500-1000 steps) or "c3"
+    (grids 100x200 .. 400x800, 4-64 samples, 5-40 shots, 1000-3000 steps);
+    velocity samples around a smoothed model of the c0 generator at that
+    size, observed data modelled from the unsmoothed model, both prepared
+    before the stream starts.  Latency runs from submit() to the result
+    being ready."""
     import torch
     from geopvi_amd import synth
     from geopvi_amd.forward.fwi2d import solver
@@ -94,7 +99,7 @@ def serve_stream(args, rate, nreq, seed, dev):
     rng = np.random.default_rng(seed)
     reqs = []
     for i in range(nreq):
-        p, m = synth.serving_request(rng, mix="bench")
+        p, m = synth.serving_request(rng, mix=mix)
         vt = synth.gradient_anomaly_model(p.nz, p.nx, p.dx, p.dz, seed=seed + i)
         vel = synth.gaussian_samples(synth.smooth(vt, 8.0), 50.0, m, seed=seed + 1000 + i).astype(np.float32)
         obs = solver.forward_batch(p, torch.from_numpy(vt.astype(np.float32).reshape(1, -1)).to(dev))[0]
@@ -113,14 +118,20 @@ def serve_stream(args, rate, nreq, seed, dev):
             if delay > 0:
                 time.sleep(delay)
             futs.append(srv.submit(p, vel, obs))
-        res = [f.result() for f in futs]
+        res = []
+        for i, f in enumerate(futs):
+            res.append(f.result())
+            if mix != "bench":
+                log(f"{mix} request {i + 1}/{nreq} done ({res[-1].latency:.1f} s)")
         t_end = max(r.finished for r in res)
         arena_waits = srv.arena_waits
     lat = np.array([r.latency for r in res])
     wait = np.array([r.admitted - r.submitted for r in res])
     span = t_end - t0
     n_full = sum(1 for r in res if r.extra.get("ckpt_interval") == 0)
+    n_split = sum(1 for r in res if r.extra.get("chunks", 1) > 1)
     return {"requests": nreq, "rate_req_s": rate, "store": f"auto: {n_full} full store, {nreq - n_full} checkpointed",
+            "split_requests": n_split,
             "arena_waits": arena_waits,
             "p50_ms": round(float(np.percentile(lat, 50)) * 1e3, 1),
             "p99_ms": round(float(np.percentile(lat, 99)) * 1e3, 1),
@@ -471,6 +482,10 @@ def main():
             serve["sub_saturation"] = serve_stream(args, args.serve_low_rate, args.serve_low_requests,
                                                    rng_seed + 400, dev)
         out["serving"] = serve
+    if args.serve_c3 > 0:
+        out["serving_c3"] = {"mix": "c3 (grids 100x200..400x800, 4-64 samples, 5-40 shots, 1000-3000 steps)",
+                             **serve_stream(args, args.serve_rate, args.serve_c3, rng_seed + 700, dev, mix="c3")}
+        log(f"c3 stream of {args.serve_c3} requests done")
 
     if not args.no_vi:
         out["vi"] = vi_leg(args, dev)
