@@ -133,15 +133,13 @@ __device__ __forceinline__ void bstore(float v, Rsrc r, unsigned voff, int soff)
 }
 
 // Order-2 transmitting extrapolation (fdtd2D_modelling.c:840-846): q1..q3
-// are p1 at 0, 1, 2 cells inward, a0..a4 are p0 at 0..4 cells inward, read
-// through p0's resource at lane offset vo and row offsets ro + j*dstep
-// (dstep: the byte step one cell inward, +-4 along x or +-pitch*4 along z;
-// the margins keep every offset non-negative).
-__device__ __forceinline__ float abc_extrapolate(float q1, float q2, float q3, Rsrc p0, unsigned vo, int ro,
-                                                 int dstep, float t11, float t12, float t13)
+// are p1 at 0, 1, 2 cells inward, a0..a4 are p0 at 0..4 cells inward.
+// abc_extrapolate reads a0..a4 through p0's resource at lane offset vo and
+// row offsets ro + j*dstep (dstep: the byte step one cell inward, +-4 along
+// x or +-pitch*4 along z; the margins keep every offset non-negative).
+__device__ __forceinline__ float abc_extrap(float q1, float q2, float q3, float a0, float a1, float a2, float a3,
+                                            float a4, float t11, float t12, float t13)
 {
-    const float a0 = bload(p0, vo, ro), a1 = bload(p0, vo, ro + dstep), a2 = bload(p0, vo, ro + 2 * dstep);
-    const float a3 = bload(p0, vo, ro + 3 * dstep), a4 = bload(p0, vo, ro + 4 * dstep);
     float h = 2.f * t11 * q1;
     h = h + 2.f * t12 * q2;
     h = h + 2.f * t13 * q3;
@@ -151,6 +149,14 @@ __device__ __forceinline__ float abc_extrapolate(float q1, float q2, float q3, R
     h = h - 2.f * t12 * t13 * a3;
     h = h - t13 * t13 * a4;
     return h;
+}
+
+__device__ __forceinline__ float abc_extrapolate(float q1, float q2, float q3, Rsrc p0, unsigned vo, int ro,
+                                                 int dstep, float t11, float t12, float t13)
+{
+    const float a0 = bload(p0, vo, ro), a1 = bload(p0, vo, ro + dstep), a2 = bload(p0, vo, ro + 2 * dstep);
+    const float a3 = bload(p0, vo, ro + 3 * dstep), a4 = bload(p0, vo, ro + 4 * dstep);
+    return abc_extrap(q1, q2, q3, a0, a1, a2, a3, a4, t11, t12, t13);
 }
 
 // w*H + (1.0 - w)*U: float product, double blend, float store (fdtd.c:840-847).
@@ -220,11 +226,12 @@ __device__ __forceinline__ StepCtx make_step(const UnitDesc& u, int mode, int it
 // lane of the wave is an inner-grid point away from the absorbing bands.
 // rp0/vo/ro: p0 of this step's resource with this lane's and this row's
 // byte offsets (inward reads of the ABC).
-template <int LC, bool INSIDE>
+template <int LC, bool INSIDE, bool BAND = false>
 __device__ __forceinline__ float point_value(const UnitDesc& u, const PointConsts& kc, const StepCtx& c, int ix, int iz,
                                              const float (&zn)[2 * LC - 1], const float* sp, float p0c, float v,
                                              bool src_row, bool rec_row, Rsrc rp0, unsigned vo, int ro,
-                                             bool active = true)
+                                             bool active = true, const float* s0 = nullptr, float wl = 0.f,
+                                             float wr = 0.f)
 {
     constexpr int H = LC - 1;
     const float hi = kc.hi;
@@ -257,16 +264,30 @@ __device__ __forceinline__ float point_value(const UnitDesc& u, const PointConst
             const float t13 = (sc * (sc - 1.f)) / 2.f;
             const Rsrc rw = make_rsrc(u.wtab);
             if (top)
-                val = abc_blend(bload(rw, 0u, (iz - LC) * 4),
+                val = abc_blend(bload(rw, (unsigned)(iz - LC) * 4u, 0),
                                 abc_extrapolate(p1c, zn[H + 1], zn[H + 2], rp0, vo, ro, pstep, t11, t12, t13), val);
             if (bot)
-                val = abc_blend(bload(rw, 0u, (ntz - LC - iz - 1) * 4),
+                val = abc_blend(bload(rw, (unsigned)(ntz - LC - iz - 1) * 4u, 0),
                                 abc_extrapolate(p1c, zn[H - 1], zn[H - 2], rp0, vo, ro, -pstep, t11, t12, t13), val);
-            if (lef)
-                val = abc_blend(bload(rw, (unsigned)(ix - LC) * 4u, 0), abc_extrapolate(p1c, sp[1], sp[2], rp0, vo, ro, 4, t11, t12, t13), val);
-            if (rig)
-                val = abc_blend(bload(rw, (unsigned)(ntx - LC - ix - 1) * 4u, 0), abc_extrapolate(p1c, sp[-1], sp[-2], rp0, vo, ro, -4, t11, t12, t13),
-                                val);
+            if (BAND) {
+                // Side-band items: p0 of the lane's row is staged in LDS with
+                // 4 columns on each side (s0 = this lane's column) and the
+                // lane's x taper weights are read once per item.
+                if (lef)
+                    val = abc_blend(wl, abc_extrap(p1c, sp[1], sp[2], s0[0], s0[1], s0[2], s0[3], s0[4], t11, t12, t13),
+                                    val);
+                if (rig)
+                    val = abc_blend(wr, abc_extrap(p1c, sp[-1], sp[-2], s0[0], s0[-1], s0[-2], s0[-3], s0[-4], t11, t12,
+                                                   t13),
+                                    val);
+            } else {
+                if (lef)
+                    val = abc_blend(bload(rw, (unsigned)(ix - LC) * 4u, 0),
+                                    abc_extrapolate(p1c, sp[1], sp[2], rp0, vo, ro, 4, t11, t12, t13), val);
+                if (rig)
+                    val = abc_blend(bload(rw, (unsigned)(ntx - LC - ix - 1) * 4u, 0),
+                                    abc_extrapolate(p1c, sp[-1], sp[-2], rp0, vo, ro, -4, t11, t12, t13), val);
+            }
         }
     }
     const float dt = kc.dt;
@@ -326,11 +347,15 @@ __device__ __forceinline__ void store_point(const UnitDesc& u, const PointConsts
     bstore(val, c.out, vo, ro);
 }
 
-// LDS floats one wavefront uses: TB rows of a row strip (64 + 2H each) or
-// the RS row slots of a side-band item (W + 2H each, RS*W = 64, RS <= 4).
+// LDS floats one wavefront uses: the rows of a row strip (64 + 2H each;
+// kRowBlock rows for a single step, whose interior blocks stage a block at
+// once, TB rows for a pair) or the RS row slots of a side-band item (p1 in
+// W + 2H and p0 in W + 8 floats each, RS*W = 64, RS <= 4).
 __host__ __device__ constexpr int wave_lds_floats(int lc, int tb)
 {
-    return tb * (64 + 2 * (lc - 1)) > 64 + 8 * (lc - 1) ? tb * (64 + 2 * (lc - 1)) : 64 + 8 * (lc - 1);
+    return (tb == 1 ? kRowBlock : tb) * (64 + 2 * (lc - 1)) > 128 + 4 * (2 * (lc - 1) + 8)
+               ? (tb == 1 ? kRowBlock : tb) * (64 + 2 * (lc - 1))
+               : 128 + 4 * (2 * (lc - 1) + 8);
 }
 
 // One side-band item (one time step): columns [0, W) (item & 1 == 0) or
@@ -374,6 +399,18 @@ __device__ __forceinline__ void process_band(const UnitDesc& u, const PointConst
     const bool inner_x = ix >= pml && ix < ntx - pml;
     float* myrow = lds + j * RWb;
     const unsigned li = cl + H;
+    // p0 of the lane's row in LDS, columns col0 - 4 .. col0 + W + 3 (the x
+    // extrapolation reads up to 4 cells inward); lanes cl < 8 also load the
+    // 4 columns on each side.
+    constexpr int RW0 = W + 8;
+    float* myrow0 = lds + RS * RWb + j * RW0;
+    const bool halo0_lane = cl < 8;
+    const unsigned hslot0 = cl < 4 ? cl : W + cl;
+    const unsigned hb0 = (unsigned)((rb + kPadRows) * pitch4 + (col0 - 4 + (int)hslot0) * 4);
+    // x taper weights of the lane's column (fdtd.c:870-900), fixed for the item
+    const Rsrc rw = make_rsrc(u.wtab);
+    const float wl = (ix >= LC && ix < pml) ? bload(rw, (unsigned)(ix - LC) * 4u, 0) : 0.f;
+    const float wr = (ix >= ntx - pml && ix < ntx - LC) ? bload(rw, (unsigned)(ntx - LC - ix - 1) * 4u, 0) : 0.f;
 
     // w[m]: p1 at row (lane's row) - H + m; the lane's row at step g is rb + H + 4 + RS*g
     float w[2 * H + 1];
@@ -381,17 +418,19 @@ __device__ __forceinline__ void process_band(const UnitDesc& u, const PointConst
     for (int m = 0; m <= 2 * H; ++m) w[m] = bload(rp1, lb, (4 + m) * pitch4);
     float hal = halo_lane ? bload(rp1, hb, (H + 4) * pitch4) : 0.f;
     float p0c = bload(rp0, lb, (H + 4) * pitch4);
+    float hal0 = halo0_lane ? bload(rp0, hb0, (H + 4) * pitch4) : 0.f;
     float vc = bload(rv, lb, (H + 4) * pitch4);
     const int ng = (ze - zb + RS - 1) / RS;
     for (int g = 0; g < ng; ++g) {
         const int base = RS * g;
         const bool more = g + 1 < ng;
-        float nw[RS], nh = 0.f, n0 = 0.f, nv = 0.f;
+        float nw[RS], nh = 0.f, n0 = 0.f, nh0 = 0.f, nv = 0.f;
         if (more) {
 #pragma unroll
             for (int k = 0; k < RS; ++k) nw[k] = bload(rp1, lb, (base + 2 * H + 5 + k) * pitch4);
             nh = halo_lane ? bload(rp1, hb, (base + RS + H + 4) * pitch4) : 0.f;
             n0 = bload(rp0, lb, (base + RS + H + 4) * pitch4);
+            nh0 = halo0_lane ? bload(rp0, hb0, (base + RS + H + 4) * pitch4) : 0.f;
             nv = bload(rv, lb, (base + RS + H + 4) * pitch4);
         } else {
 #pragma unroll
@@ -400,12 +439,14 @@ __device__ __forceinline__ void process_band(const UnitDesc& u, const PointConst
         const int iz = zb + base + j;
         myrow[li] = w[H];
         if (halo_lane) myrow[hslot] = hal;
+        myrow0[cl + 4] = p0c;
+        if (halo0_lane) myrow0[hslot0] = hal0;
         __builtin_amdgcn_wave_barrier();
         if (iz < ze) {
             const bool rec_row = iz == r_iz, src_row = iz == s_iz;
             const unsigned vo = lb + (unsigned)((base + H + 4) * pitch4 - ro_abc);
-            const float val = point_value<LC, false>(u, kc, c, ix, iz, w, myrow + li, p0c, vc, src_row, rec_row, rp0,
-                                                     vo, ro_abc);
+            const float val = point_value<LC, false, true>(u, kc, c, ix, iz, w, myrow + li, p0c, vc, src_row, rec_row,
+                                                           rp0, vo, ro_abc, true, myrow0 + cl + 4, wl, wr);
             const bool inner = inner_x && iz >= pml && iz < ntz - pml;
             const unsigned gv = inner ? (unsigned)(((iz - pml) * nx + (ix - pml)) * 4) : 0u;
             store_point(u, kc, c, val, p0c, w[H], vc, ix, vo, ro_abc, gv, 0, rg, rec_row, inner);
@@ -417,6 +458,7 @@ __device__ __forceinline__ void process_band(const UnitDesc& u, const PointConst
         for (int k = 0; k < RS; ++k) w[2 * H + 1 - RS + k] = nw[k];
         hal = nh;
         p0c = n0;
+        hal0 = nh0;
         vc = nv;
     }
 }
@@ -433,7 +475,7 @@ __device__ __forceinline__ void process_band(const UnitDesc& u, const PointConst
 //   recomputed bit-identically by the neighbouring strips and segments.
 //
 // TB (1 or 2) is the unit's steps per macro step, UnitDesc::tb.  `lds` is
-// the wave's private scratch: TB rows of 64 + 2H floats.
+// the wave's private scratch: rows of 64 + 2H floats (wave_lds_floats).
 template <int LC, int TB>
 __device__ __forceinline__ void process_item(const UnitDesc& u, int s, int item, float* lds)
 {
@@ -484,7 +526,9 @@ __device__ __forceinline__ void process_item(const UnitDesc& u, int s, int item,
     const unsigned vo = li * 4u;
     const bool halo_lane = lane < 2 * H;
     const unsigned hslot = lane < H ? lane : 64 + lane;
-    const unsigned ho = hslot * 4u;
+    // Lanes that are not halo lanes read the first halo lane's word (same
+    // cache line) so the halo loads need no execution mask.
+    const unsigned ho = halo_lane ? hslot * 4u : 0u;
     const int pitch4 = pitch * 4;
     auto row_off = [&](int z) { return (z + kPadRows) * pitch4 + cx0 * 4; };
     // Every owned lane of the wave inner and away from the bands (for both
@@ -520,7 +564,7 @@ __device__ __forceinline__ void process_item(const UnitDesc& u, int s, int item,
 #pragma unroll
     for (int r = 0; r < R; ++r) {
         const int ro = row_off(za_begin + r);
-        hal[r] = halo_lane ? bload(rp1, ho, ro) : 0.f;
+        hal[r] = bload(rp1, ho, ro);
         p0r[r] = bload(rp0, vo, ro);
         vh[H + r] = bload(rv, vo, ro);
     }
@@ -535,7 +579,7 @@ __device__ __forceinline__ void process_item(const UnitDesc& u, int s, int item,
             for (int r = 0; r < R; ++r) {
                 const int ro = row_off(za1 + r);
                 n1[r] = bload(rp1, vo, row_off(za1 + H + r));
-                nh[r] = halo_lane ? bload(rp1, ho, ro) : 0.f;
+                nh[r] = bload(rp1, ho, ro);
                 n0[r] = bload(rp0, vo, ro);
                 nv[r] = bload(rv, vo, ro);
             }
@@ -544,6 +588,37 @@ __device__ __forceinline__ void process_item(const UnitDesc& u, int s, int item,
             for (int r = 0; r < R; ++r) n1[r] = nh[r] = n0[r] = nv[r] = 0.f;
         }
 
+        // A block of R interior rows (single step, every owned lane inner,
+        // no source or receiver row) takes the lean path: its R rows are
+        // staged in LDS at once and updated with no per-row tests.  Imaging
+        // steps keep the per-row path, where each row's loads of the stored
+        // field and the gradient are issued ahead of that row's arithmetic.
+        if (TB == 1 && strip_inside && !(cA.has_save && cA.mode == kAdj) && za0 >= pml && za0 + R <= za_end &&
+            za0 + R <= ntz - pml && (unsigned)(r_iz - za0) >= (unsigned)R && (unsigned)(s_iz - za0) >= (unsigned)R) {
+#pragma unroll
+            for (int r = 0; r < R; ++r) lds[r * RW + li] = w1[r + H];
+            if (halo_lane) {
+#pragma unroll
+                for (int r = 0; r < R; ++r) lds[r * RW + hslot] = hal[r];
+            }
+            __builtin_amdgcn_wave_barrier();
+            float val[R];
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                float zA[2 * H + 1];
+#pragma unroll
+                for (int m = 0; m <= 2 * H; ++m) zA[m] = w1[r + m];
+                val[r] = point_value<LC, true>(u, kc, cA, ix, za0 + r, zA, lds + r * RW + li, p0r[r], vh[H + r], false,
+                                               false, rp0, vo, row_off(za0 + r), lane_ok);
+            }
+            __builtin_amdgcn_wave_barrier();
+            if (owned) {
+#pragma unroll
+                for (int r = 0; r < R; ++r)
+                    store_point(u, kc, cA, val[r], p0r[r], w1[r + H], vh[H + r], ix, vo, row_off(za0 + r), gvo,
+                                (za0 + r - pml) * nx4, rg, false, inner_x);
+            }
+        } else
 #pragma unroll
         for (int r = 0; r < R; ++r) {
             const int za = za0 + r;
