@@ -81,6 +81,29 @@ def algorithmic_bytes_per_unit(p) -> int:
     return p.nt * p.ntp * 32 + nsave * p.ni * 16 + p.nt * p.nr * 12
 
 
+def stream_3r1w_gbs(dev, mib: int = 512, reps: int = 10) -> float:
+    """HBM rate of a flat streaming kernel that reads three arrays and
+    writes one (torch.addcmul over `mib` MiB each), in GB/s."""
+    import torch
+    n = mib * (1 << 20) // 4
+    a = torch.rand(n, device=dev)
+    b = torch.rand(n, device=dev)
+    c = torch.rand(n, device=dev)
+    d = torch.empty(n, device=dev)
+    torch.addcmul(a, b, c, out=d)
+    torch.cuda.synchronize()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        torch.addcmul(a, b, c, out=d)
+    e1.record()
+    torch.cuda.synchronize()
+    sec = e0.elapsed_time(e1) * 1e-3 / reps
+    del a, b, c, d
+    return 4 * n * 4 / sec / 1e9
+
+
 def serve_stream(args, rate, nreq, seed, dev, mix="bench"):
     """A Poisson stream of misfit+gradient requests through the server.
 
@@ -451,6 +474,11 @@ def main():
     if tune is not None:
         out["autotune"] = {"us_per_time_step": tune["us_per_step"], "candidates": len(tune["timings"])}
     del ws
+    # What a plain streaming kernel with the step's read:write mix reaches on
+    # this GPU (torch's addcmul: 3 reads, 1 write), for scale.
+    stream = stream_3r1w_gbs(dev)
+    out["roofline"]["stream_3r1w_GBs"] = round(stream, 1)
+    out["roofline"]["frac_of_stream"] = round(achieved / stream, 4)
 
     if not args.no_plain:
         vs0 = vel[0].contiguous()
