@@ -13,7 +13,6 @@
 #include <mutex>
 #include <string>
 #include <tuple>
-#include <tuple>
 #include <vector>
 
 #include "fwi2d_kernels.h"
@@ -83,6 +82,13 @@ int validate(const gpvi_fwi2d_params* p)
     const int lc = psm ? 0 : p->Lc + 1, pml = p->pml0 + lc;
     const long ntx = (long)p->nx + 2 * pml, ntz = (long)p->nz + 2 * pml;
     if (ntx > (1 << 20) || ntz > (1 << 20)) return fail(GPVI_ERR_INVALID, "grid too large");
+    // The kernels address a field buffer (padded rows plus margins) and a
+    // shot's records with 32-bit offsets.
+    const long pitch = (ntx + kStripWidth - 1) / kStripWidth * kStripWidth;
+    if (pitch * (ntz + 2 * kPadRows) * 4 >= (1L << 31))
+        return fail(GPVI_ERR_INVALID, "grid too large: a %ldx%ld padded field must stay under 2 GiB", ntz, ntx);
+    if ((long)p->nt * p->nr >= (1L << 31))
+        return fail(GPVI_ERR_INVALID, "nt*nr=%ld: a shot's records must stay under 2^31 values", (long)p->nt * p->nr);
     const long s0 = (long)pml + p->ns0, s1 = s0 + (long)(p->ns - 1) * p->ds;
     const long r0 = (long)pml + p->nr0, r1 = r0 + (long)(p->nr - 1) * p->dr;
     const long siz = (long)pml + p->depths, riz = (long)pml + p->depthr;

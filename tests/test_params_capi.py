@@ -87,6 +87,8 @@ def test_param_file_blank_lines_and_errors(tmp_path):
     (dict(dr=0), -1),
     (dict(nt=0), -1),
     (dict(dt=0.0), -1),
+    (dict(nx=24000, nz=24000), -1),        # a padded field of 2 GiB or more
+    (dict(nt=1 << 25, nr=64), -1),         # 2^31 record values per shot
 ])
 def test_validation_codes(change, code):
     p = synth.workload_params("c0", **change)
