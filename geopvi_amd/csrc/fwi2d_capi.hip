@@ -444,9 +444,9 @@ std::map<GraphKey, hipGraphExec_t> g_graphs;
 std::map<int, hipStream_t> g_capture_streams;
 constexpr size_t kMaxGraphs = 64;
 
-int get_graph(const GraphKey& key, hipGraphExec_t* out)
+// Find or build the segment for `key`; called with g_graph_mu held.
+int get_graph_locked(const GraphKey& key, hipGraphExec_t* out)
 {
-    std::lock_guard<std::mutex> lk(g_graph_mu);
     auto it = g_graphs.find(key);
     if (it != g_graphs.end()) {
         *out = it->second;
@@ -486,6 +486,20 @@ int get_graph(const GraphKey& key, hipGraphExec_t* out)
     if (e3 != hipSuccess) return fail(GPVI_ERR_HIP, "hipGraphInstantiate: %s", hipGetErrorString(e3));
     g_graphs[key] = exec;
     *out = exec;
+    return 0;
+}
+
+// Launch the segment for `key` `reps` times on `st`.  The lookup and the
+// launches share one lock: a full cache is emptied (after a device-wide
+// synchronisation) by whichever thread next needs a new segment, so an
+// executable graph is never launched after another thread destroyed it.
+int run_graph(const GraphKey& key, hipStream_t st, int reps)
+{
+    std::lock_guard<std::mutex> lk(g_graph_mu);
+    hipGraphExec_t exec;
+    int rc = get_graph_locked(key, &exec);
+    if (rc) return rc;
+    for (int r = 0; r < reps; ++r) HIP_TRY(hipGraphLaunch(exec, st));
     return 0;
 }
 
@@ -830,10 +844,7 @@ int gpvi_fwi2d_batch(const gpvi_fwi2d_params* p, int nsample, const float* vel_i
     } else {
         const int seg = std::min(tn.graph_steps, j.total);
         GraphKey key{L.units, L.tiles, L.ctl, grid, g.tb, 1 << (g.lc - 4), seg, current_device(), {0, 0, 0, 0}};
-        hipGraphExec_t exec;
-        if ((rc = get_graph(key, &exec))) return rc;
-        const int reps = (j.total + seg - 1) / seg;
-        for (int r = 0; r < reps; ++r) HIP_TRY(hipGraphLaunch(exec, st));
+        if ((rc = run_graph(key, st, (j.total + seg - 1) / seg))) return rc;
     }
     if (phase != GPVI_PHASE_FORWARD)
         HIP_TRY(launch_grad_reduce(st, L.gpriv, L.grad_stride, grad, g.ns, g.ni, nsample));
@@ -1065,8 +1076,6 @@ int gpvi_engine_advance(gpvi_engine* e, int nsteps)
     // Items 0: every launch uses the resident grid, so the graph survives
     // changes of the job set.
     GraphKey key{e->units, e->tiles, e->ctl, e->grid, e->tb, mask, e->seg_steps, e->device, {0, 0, 0, 0}};
-    hipGraphExec_t exec;
-    if ((rc = get_graph(key, &exec))) return rc;
     long long pending = 0;  // never run past the last active job's last step
     for (const auto& jb : e->active) pending = std::max(pending, jb.end_step - e->gstep);
     const long long todo = std::min<long long>(nsteps, pending);
@@ -1078,7 +1087,7 @@ int gpvi_engine_advance(gpvi_engine* e, int nsteps)
             (void)hipEventDestroy(e->inflight.front());
             e->inflight.erase(e->inflight.begin());
         }
-        HIP_TRY(hipGraphLaunch(exec, e->stream));
+        if ((rc = run_graph(key, e->stream, 1))) return rc;
         hipEvent_t ev;
         HIP_TRY(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
         HIP_TRY(hipEventRecord(ev, e->stream));
