@@ -5,6 +5,7 @@
 #include <hipfft/hipfft.h>
 
 #include <algorithm>
+#include <climits>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -973,6 +974,9 @@ int gpvi_engine_create(void* stream, int max_units, int max_tiles, gpvi_engine**
     if (err == hipSuccess) err = hipMalloc((void**)&e->tiles, sizeof(TileRef) * (size_t)max_tiles);
     if (err == hipSuccess) err = hipMalloc((void**)&e->ctl, sizeof(StepCtl));
     if (err != hipSuccess) {
+        if (e->units) (void)hipFree(e->units);
+        if (e->tiles) (void)hipFree(e->tiles);
+        if (e->ctl) (void)hipFree(e->ctl);
         delete e;
         return fail(GPVI_ERR_HIP, "engine tables: %s", hipGetErrorString(err));
     }
@@ -1026,6 +1030,10 @@ int gpvi_engine_submit(gpvi_engine* e, const gpvi_fwi2d_params* p, int nsample, 
                                           "gpvi_fwi2d_batch");
     if (nsample < 1 || !vel_inner || !record_obs || !record_syn || !grad)
         return fail(GPVI_ERR_INVALID, "bad job arguments");
+    // The device step counter is a 32-bit int: an idle engine restarts it
+    // at 0 (the next rebuild writes it in stream order after the work
+    // already queued).
+    if (e->active.empty()) e->gstep = 0;
     EngineJob jb;
     make_geom(p, e->variant, e->tb, &jb.g);
     jb.sc = make_sched(jb.g, ckpt_interval);
@@ -1050,6 +1058,11 @@ int gpvi_engine_submit(gpvi_engine* e, const gpvi_fwi2d_params* p, int nsample, 
         return rc;
     }
     jb.j = make_job(p, jb.g, jb.sc, nsample, jb.L, record_obs, record_syn, run_fwi, GPVI_PHASE_ALL, 1);
+    if (e->gstep + jb.j.total + e->seg_steps > INT_MAX) {
+        engine_free_slots(e, first, jb.nunit);
+        (void)hipEventDestroy(jb.done);
+        return fail(GPVI_ERR_CAPACITY, "step counter at %lld: let the engine drain before submitting", e->gstep);
+    }
     jb.j.unit0 = first;
     jb.j.start = (int)e->gstep;
     jb.end_step = e->gstep + jb.j.total;
