@@ -34,6 +34,7 @@ from typing import Optional, Tuple
 import numpy as np
 import torch
 
+from . import _lib
 from .engine import Engine, EngineJob, best_ckpt_interval, job_workspace_bytes
 from .forward.fwi2d import solver
 from .forward.fwi2d.params import FWIParams
@@ -337,24 +338,40 @@ class FWIServer:
                 self.arena.release(off, need)
                 if self._inflight:
                     return
-                # nothing running will free memory: fail the request (all its
-                # queued chunks, if it was split)
-                self._queue.popleft()
-                if sp is not None:
-                    self._queue = collections.deque(r for r in self._queue if r.split is not sp)
-                if not req.future.done():
-                    req.future.set_exception(MemoryError(f"no room for the outputs: {exc}"))
+                # nothing running will free memory
+                self._fail_queued(req, MemoryError(f"no room for the outputs: {exc}"))
+                continue
+            self.stream.wait_event(req.ready)
+            try:
+                self.engine.submit(job)
+            except _lib.GpviError as exc:
+                if exc.code != _lib.GPVI_ERR_CAPACITY:
+                    raise
+                # The engine's unit slots (a contiguous run per job) or tile
+                # list are full: wait for running jobs, or fail the request
+                # when nothing runs.
+                self.arena.release(off, need)
+                if self._inflight:
+                    return
+                self._fail_queued(req, exc)
                 continue
             self._queue.popleft()
             req.ckpt, req.need, req.extra["arena_offset"] = ckpt, need, off
-            self.stream.wait_event(req.ready)
-            self.engine.submit(job)
             req.job = job
             req.admitted = time.perf_counter()
             if req.split is not None and not req.split.admitted:
                 req.split.admitted = req.admitted
             self._inflight[job.id] = req
             self._units_in_use += units
+
+    def _fail_queued(self, req, exc):
+        """Drop the request at the head of the queue (with all its queued
+        chunks, if it was split) and fail its future."""
+        self._queue.popleft()
+        if req.split is not None:
+            self._queue = collections.deque(r for r in self._queue if r.split is not req.split)
+        if not req.future.done():
+            req.future.set_exception(exc)
 
     def _reap(self):
         done = [jid for jid in self._inflight if self.engine.status(jid) == 1]

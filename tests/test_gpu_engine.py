@@ -183,3 +183,22 @@ def test_server_splits_requests_larger_than_half_the_arena(three_jobs):
                        torch.zeros(p.record_len, device="cuda"))
     assert res.extra["chunks"] == 3
     assert torch.equal(res.record_syn, syn) and torch.equal(res.grad, grad) and torch.equal(res.misfit, mis)
+
+
+def test_server_refuses_a_request_beyond_the_engine_tables(three_jobs):
+    """A request with more (sample, shot) units than the engine's slot table
+    fails on its own with the engine's capacity error; the server keeps
+    serving, and the next request gets the standalone batch's bits."""
+    from geopvi_amd import _lib
+    from geopvi_amd.serving import FWIServer
+    p, v, o = three_jobs[0]
+    units = v.shape[0] * p.ns
+    v1 = v[:1].contiguous()
+    syn, grad, mis = (t.clone() for t in solver.fwi_batch(p, v1, o))
+    torch.cuda.synchronize()
+    with FWIServer(steps_per_tick=32, max_units=units - 1) as srv:
+        with pytest.raises(_lib.GpviError) as info:
+            srv.submit(p, v, o).result(timeout=120)
+        assert info.value.code == _lib.GPVI_ERR_CAPACITY
+        res = srv.submit(p, v1, o).result(timeout=120)
+    assert torch.equal(res.record_syn, syn) and torch.equal(res.grad, grad) and torch.equal(res.misfit, mis)
