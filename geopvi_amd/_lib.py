@@ -66,6 +66,7 @@ class GpviError(RuntimeError):
     def __init__(self, code: int, msg: str):
         super().__init__(f"[{code}] {msg}")
         self.code = code
+        self.msg = msg
 
 
 def load(path: str = LIB_PATH):

@@ -350,10 +350,11 @@ class FWIServer:
                 # The engine's unit slots (a contiguous run per job) or tile
                 # list are full: wait for running jobs, or fail the request
                 # when nothing runs.
+                job.workspace = None
                 self.arena.release(off, need)
                 if self._inflight:
                     return
-                self._fail_queued(req, exc)
+                self._fail_queued(req, _lib.GpviError(exc.code, exc.msg))
                 continue
             self._queue.popleft()
             req.ckpt, req.need, req.extra["arena_offset"] = ckpt, need, off
@@ -366,7 +367,12 @@ class FWIServer:
 
     def _fail_queued(self, req, exc):
         """Drop the request at the head of the queue (with all its queued
-        chunks, if it was split) and fail its future."""
+        chunks, if it was split) and fail its future with `exc`.  `exc` must
+        be a fresh exception with no traceback or context: one raised here
+        would keep this thread's frames, and with them an arena view, alive
+        through the future it is stored in."""
+        exc.__context__ = None
+        exc.__traceback__ = None
         self._queue.popleft()
         if req.split is not None:
             self._queue = collections.deque(r for r in self._queue if r.split is not req.split)

@@ -196,9 +196,14 @@ def test_server_refuses_a_request_beyond_the_engine_tables(three_jobs):
     v1 = v[:1].contiguous()
     syn, grad, mis = (t.clone() for t in solver.fwi_batch(p, v1, o))
     torch.cuda.synchronize()
+    base = torch.cuda.memory_allocated()
     with FWIServer(steps_per_tick=32, max_units=units - 1) as srv:
         with pytest.raises(_lib.GpviError) as info:
             srv.submit(p, v, o).result(timeout=120)
         assert info.value.code == _lib.GPVI_ERR_CAPACITY
         res = srv.submit(p, v1, o).result(timeout=120)
     assert torch.equal(res.record_syn, syn) and torch.equal(res.grad, grad) and torch.equal(res.misfit, mis)
+    # the failed request's exception holds no reference to the arena: closing
+    # the server returns it while the exception is still alive
+    assert info.value.__traceback__ is None or info.value.__context__ is None
+    assert torch.cuda.memory_allocated() - base < (64 << 20)
