@@ -85,7 +85,7 @@ int validate(const gpvi_fwi2d_params* p)
     if (ntx > (1 << 20) || ntz > (1 << 20)) return fail(GPVI_ERR_INVALID, "grid too large");
     // The kernels address a field buffer (padded rows plus margins) and a
     // shot's records with 32-bit offsets.
-    const long pitch = (ntx + kStripWidth - 1) / kStripWidth * kStripWidth;
+    const long pitch = (ntx + 31 + kStripWidth - 1) / kStripWidth * kStripWidth;  // with the row shift
     if (pitch * (ntz + 2 * kPadRows) * 4 >= (1L << 31))
         return fail(GPVI_ERR_INVALID, "grid too large: a %ldx%ld padded field must stay under 2 GiB", ntz, ntx);
     if ((long)p->nt * p->nr >= (1L << 31))
@@ -160,9 +160,11 @@ bool fastdiv_ok(float b, float* recip)
     return ok;
 }
 
-// Field buffers: ntz rows of `pitch` floats (pitch = ntx rounded up to the
-// 64-column strip width) with kPadRows margin rows before and after; `fe` is
-// the buffer stride and `origin` the offset of row 0 inside a buffer.
+// Field buffers: ntz rows of `pitch` floats with kPadRows margin rows before
+// and after; every row starts xoff floats into its pitch (xoff < 32, so the
+// first row-strip column is 128-byte aligned) and pitch = ntx + xoff rounded
+// up to the 64-column strip width.  `fe` is the buffer stride and `origin`
+// the offset of row 0, column 0 inside a buffer.
 struct Geom {
     int lc, pml, ntx, ntz, pitch, nx, nz, ni, nt, k, nr, ns, nsave, pml0;
     int variant, tile_x, tile_z, tiles_x, tiles_z, tpu, zseg, tb, stripw, band_w, band_tz;
@@ -180,12 +182,9 @@ void make_geom(const gpvi_fwi2d_params* p, int variant, int tb, Geom* g)
     g->pml = p->pml0 + g->lc;
     g->ntx = p->nx + 2 * g->pml;
     g->ntz = p->nz + 2 * g->pml;
-    g->pitch = (g->ntx + kStripWidth - 1) / kStripWidth * kStripWidth;
     g->nx = p->nx; g->nz = p->nz; g->ni = p->nx * p->nz;
     g->nt = p->nt; g->k = p->nt_interval; g->nr = p->nr; g->ns = p->ns;
     g->nsave = (p->nt + p->nt_interval - 1) / p->nt_interval;
-    g->fe = (int64_t)g->pitch * (g->ntz + 2 * kPadRows);
-    g->origin = (int64_t)g->pitch * kPadRows;
     g->variant = variant;
     variant_tile(variant, &g->tile_x, &g->tile_z);
     g->zseg = g->tile_z;
@@ -224,6 +223,13 @@ void make_geom(const gpvi_fwi2d_params* p, int variant, int tb, Geom* g)
         }
     }
     g->tpu = 2 * g->band_tz + g->tiles_x * g->tiles_z;
+    // Row strips start at column w_left: shift every row by xoff floats so
+    // that column lands on a 128-byte cache line (32 floats).
+    const int wl0 = g->band_w & 0xff;
+    const int xoff = (32 - wl0 % 32) % 32;
+    g->pitch = (g->ntx + xoff + kStripWidth - 1) / kStripWidth * kStripWidth;
+    g->fe = (int64_t)g->pitch * (g->ntz + 2 * kPadRows);
+    g->origin = (int64_t)g->pitch * kPadRows + xoff;
     g->pml0 = p->pml0;
     fd_coeffs(g->lc, g->fd);
     const bool okx = fastdiv_ok(p->dx, &g->rdx);

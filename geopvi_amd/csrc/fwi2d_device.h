@@ -56,7 +56,9 @@ __host__ __device__ constexpr float fd_weight(int lc, int m)
 enum StepMode : int { kFwd = 0, kRec = 1, kAdj = 2, kIdle = 3 };
 
 // Device unit descriptor.  Field buffers are "pitched": ntz rows of `pitch`
-// floats (pitch >= ntx, a multiple of 64 floats).
+// floats (pitch >= ntx, a multiple of 64 floats); each row may start a few
+// floats into its pitch, so that the row strips' first column is 128-byte
+// aligned.
 struct UnitDesc {
     int ntx, ntz, pitch, nx;
     int nz, pml, pml0, lc;
@@ -77,8 +79,8 @@ struct UnitDesc {
     float fd[8];
     int64_t field_elems;  // stride between field buffers: (ntz + 2*kPadRows) * pitch
     const float* wtab;    // ABC taper w[i] = 1 - i/pml0, i < pml0
-    // Field pointers point at row 0 of a buffer that has kPadRows rows of
-    // margin before and after, so every row a strip segment touches is
+    // Field pointers point at row 0, column 0 of a buffer that has kPadRows
+    // rows of margin before and after, so every row a strip segment touches is
     // addressable without bounds checks (margin values never reach a grid
     // point of the solution).
     float* fwd;           // 4 field buffers
