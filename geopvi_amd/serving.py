@@ -197,7 +197,7 @@ class FWIServer:
         self._stop = False
         self._error: Optional[BaseException] = None
         self.completed: list = []
-        self.arena_waits = 0  # admissions deferred until running jobs free arena space
+        self.arena_waits = 0  # requests (or chunks) whose admission waited for arena space
         self._thread = threading.Thread(target=self._run, name="fwi-server", daemon=True)
         self._thread.start()
 
@@ -318,7 +318,9 @@ class FWIServer:
             if placed is None:
                 # every job's checkpointed workspace fits the empty arena
                 # (submit() checks), so running jobs will make room
-                self.arena_waits += 1
+                if not req.extra.get("waited"):
+                    req.extra["waited"] = True
+                    self.arena_waits += 1
                 return
             ckpt, need, off = placed
             outputs = None
