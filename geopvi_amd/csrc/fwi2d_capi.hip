@@ -767,6 +767,48 @@ int psm_plans(int ntz, int ntx, int batch, PsmPlans* out)
     return 0;
 }
 
+// Layout stamps: what a GPVI_PHASE_FORWARD call laid out in a workspace,
+// keyed by the workspace address, so that the GPVI_PHASE_ADJOINT call on the
+// same workspace uses the same layout (or fails) even if the tuning changed in
+// between.  A later forward call on the same address replaces the stamp.
+struct LayoutStamp {
+    gpvi_fwi2d_params p;
+    int nsample, ckpt, variant, tb;
+    unsigned long long seq;
+};
+std::mutex g_stamp_mu;
+std::map<const void*, LayoutStamp> g_stamps;
+unsigned long long g_stamp_seq = 0;
+constexpr size_t kMaxStamps = 4096;
+
+void layout_stamp_put(const void* ws, const gpvi_fwi2d_params* p, int nsample, int ckpt, const Tuning& tn)
+{
+    std::lock_guard<std::mutex> lk(g_stamp_mu);
+    if (g_stamps.size() >= kMaxStamps && !g_stamps.count(ws)) {
+        auto oldest = g_stamps.begin();
+        for (auto it = g_stamps.begin(); it != g_stamps.end(); ++it)
+            if (it->second.seq < oldest->second.seq) oldest = it;
+        g_stamps.erase(oldest);
+    }
+    g_stamps[ws] = LayoutStamp{*p, nsample, ckpt, tn.variant, tn.tb, ++g_stamp_seq};
+}
+
+int layout_stamp_get(const void* ws, const gpvi_fwi2d_params* p, int nsample, int ckpt, Tuning* tn)
+{
+    std::lock_guard<std::mutex> lk(g_stamp_mu);
+    auto it = g_stamps.find(ws);
+    if (it == g_stamps.end())
+        return fail(GPVI_ERR_INVALID, "adjoint phase: this workspace was not prepared by a GPVI_PHASE_FORWARD call "
+                                      "(or its record was evicted by %zu later forward calls)", kMaxStamps);
+    const LayoutStamp& s = it->second;
+    if (std::memcmp(&s.p, p, sizeof *p) != 0 || s.nsample != nsample || s.ckpt != ckpt)
+        return fail(GPVI_ERR_INVALID, "adjoint phase: parameters, nsample or ckpt_interval differ from the forward "
+                                      "phase that prepared this workspace");
+    tn->variant = s.variant;
+    tn->tb = s.tb;
+    return 0;
+}
+
 int psm_run(hipStream_t st, const Geom& g, const Sched& sc, const JobDesc& j, const BatchLayout& L, int nunit)
 {
     HIP_TRY(launch_psm_wavenumber(st, L.ktab, g.ntx, g.ntz, j.dx, j.dz));
@@ -813,7 +855,14 @@ int gpvi_fwi2d_batch(const gpvi_fwi2d_params* p, int nsample, const float* vel_i
     if (!record_obs || !record_syn || (phase != GPVI_PHASE_ADJOINT && !vel_inner) ||
         (phase != GPVI_PHASE_FORWARD && !grad))
         return fail(GPVI_ERR_INVALID, "NULL buffer");
-    const Tuning tn = tuning();
+    Tuning tn = tuning();
+    // The adjoint phase continues the forward phase's workspace: it must use
+    // the layout that phase used (tile variant and steps per launch fix the
+    // strip width, band items, row shift, pitch and field stride), whatever
+    // the process-wide tuning says now.
+    rc = phase == GPVI_PHASE_ADJOINT ? layout_stamp_get(workspace, p, nsample, ckpt_interval, &tn)
+                                     : 0;
+    if (rc) return rc;
     Geom g;
     make_geom(p, tn.variant, tn.tb, &g);
     if (g.lc == 0 && (int64_t)nsample * g.ns > 65535)  // one grid row of launches per unit
@@ -828,6 +877,7 @@ int gpvi_fwi2d_batch(const gpvi_fwi2d_params* p, int nsample, const float* vel_i
     if (phase != GPVI_PHASE_ADJOINT) {
         if ((rc = job_setup(st, p, g, vel_inner, L.vel, L.wav, L.zero, nsample))) return rc;
     }
+    if (phase == GPVI_PHASE_FORWARD && !model_only) layout_stamp_put(workspace, p, nsample, ckpt_interval, tn);
     const int store = (phase == GPVI_PHASE_FORWARD && (flags & GPVI_FLAG_NO_STORE)) ? 0 : 1;
     JobDesc j = make_job(p, g, sc, nsample, L, record_obs, record_syn, run_fwi, phase, store);
     HIP_TRY(launch_build_units(st, j, L.units, L.tiles));

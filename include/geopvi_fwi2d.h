@@ -38,7 +38,10 @@ extern "C" {
 /* Phases of a batched evaluation. */
 #define GPVI_PHASE_ALL 0      /* forward + adjoint in one call */
 #define GPVI_PHASE_FORWARD 1  /* forward only: records, misfit, checkpoints kept in the workspace */
-#define GPVI_PHASE_ADJOINT 2  /* adjoint from a workspace left by GPVI_PHASE_FORWARD */
+#define GPVI_PHASE_ADJOINT 2  /* adjoint from a workspace left by GPVI_PHASE_FORWARD: same parameters,
+                                 nsample and ckpt_interval, and the launch layout that call used (a later
+                                 change of tuning or time blocking does not apply); GPVI_ERR_INVALID if the
+                                 workspace was not prepared by such a call */
 
 /* Flags of gpvi_fwi2d_batch. */
 #define GPVI_FLAG_NO_GRAPH 1  /* one launch per step instead of replayed graph segments */

@@ -33,7 +33,27 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 # GPVI_ORACLE_LIB points the checker at another build of the same source
 # (used to run the pin tests against deliberately mutated oracles).
 _LIB_PATH = os.environ.get("GPVI_ORACLE_LIB") or os.path.join(_HERE, "libfwi2d_oracle.so")
+# The same arithmetic with row-major grid sweeps (identical results, several
+# times faster): for long fixture runs.  select(row_major=True) switches.
+_LIB_PATH_RM = os.path.join(_HERE, "libfwi2d_oracle_rm.so")
 _lib = None
+_active = _LIB_PATH
+
+
+def select(row_major: bool = False) -> str:
+    """Use the reference-order build (default) or the row-major build."""
+    return use_library(_LIB_PATH_RM if row_major else _LIB_PATH)
+
+
+def use_library(path: str) -> str:
+    """Route every call to the oracle build at ``path`` (another build of
+    the same source, e.g. a deliberately mutated one in a mutation test);
+    returns the path in use before."""
+    global _lib, _active
+    prev = _active
+    if path != _active:
+        _active, _lib = path, None
+    return prev
 
 
 class OParams(ctypes.Structure):
@@ -53,9 +73,9 @@ def build() -> str:
 def lib():
     global _lib
     if _lib is None:
-        if not os.path.exists(_LIB_PATH):
-            raise RuntimeError(f"oracle library missing: {_LIB_PATH} (run `make -C oracle`)")
-        L = ctypes.CDLL(_LIB_PATH)
+        if not os.path.exists(_active):
+            raise RuntimeError(f"oracle library missing: {_active} (run `make -C oracle`)")
+        L = ctypes.CDLL(_active)
         fp = ctypes.POINTER(ctypes.c_float)
         dp = ctypes.POINTER(ctypes.c_double)
         pp = ctypes.POINTER(OParams)

@@ -96,6 +96,21 @@ int ofwi_fd_coefficient(int lc, float *fd)
     }
 }
 
+/* Loop order of the grid sweeps.  The reference walks its row-major arrays
+ * ix-outer (fdtd.c:768 and every sweep after it), and the default build keeps
+ * that order so that it is a faithful CPU timing baseline.  Every point of a
+ * sweep is computed from values the sweep does not write, so the order
+ * changes no result: OFWI_ROW_MAJOR builds the same arithmetic iz-outer
+ * (libfwi2d_oracle_rm.so, a faster checker for long fixture runs; the tests
+ * hold the two builds to identical bits). */
+#ifdef OFWI_ROW_MAJOR
+#define OFWI_XZ(ix, x0, x1, iz, z0, z1) \
+    for (int iz = (z0); iz < (z1); ++iz) for (int ix = (x0); ix < (x1); ++ix)
+#else
+#define OFWI_XZ(ix, x0, x1, iz, z0, z1) \
+    for (int ix = (x0); ix < (x1); ++ix) for (int iz = (z0); iz < (z1); ++iz)
+#endif
+
 /* float build: reference semantics */
 #define REAL float
 #define SUF(x) x##_f32

@@ -29,12 +29,10 @@ void SUF(ofwi_pad_velocity)(int pml, int ntx, int ntz, const REAL *vin, REAL *ve
 {
     const int nx = ntx - 2 * pml;
 #pragma omp parallel for collapse(2)
-    for (int ix = pml; ix < ntx - pml; ++ix)
-        for (int iz = pml; iz < ntz - pml; ++iz)
+    OFWI_XZ(ix, pml, ntx - pml, iz, pml, ntz - pml)
             velp[(long)iz * ntx + ix] = vin[(long)(iz - pml) * nx + (ix - pml)];
 #pragma omp parallel for collapse(2)
-    for (int ix = pml; ix < ntx - pml; ++ix)
-        for (int iz = 0; iz < pml; ++iz) {
+    OFWI_XZ(ix, pml, ntx - pml, iz, 0, pml) {
             velp[(long)iz * ntx + ix] = velp[(long)pml * ntx + ix];
             velp[(long)(ntz - 1 - iz) * ntx + ix] = velp[(long)(ntz - pml - 1) * ntx + ix];
         }
@@ -66,8 +64,7 @@ static void SUF(stencil)(SUF(ofwi_ctx) *c)
     const REAL *fd = c->fd, *v = c->velp, *p0 = c->p0, *p1 = c->p1;
     REAL *p2 = c->p2;
 #pragma omp parallel for collapse(2)
-    for (int ix = L; ix < ntx - L; ++ix)
-        for (int iz = L; iz < ntz - L; ++iz) {
+    OFWI_XZ(ix, L, ntx - L, iz, L, ntz - L) {
             const long ip = (long)iz * ntx + ix;
             REAL dxx = fd[0] * p1[ip];
             for (int m = 1; m < L; ++m)
@@ -101,15 +98,13 @@ static void SUF(abc)(SUF(ofwi_ctx) *c)
     REAL *p2 = c->p2;
     const long nx_l = ntx;
 #pragma omp parallel for collapse(2)
-    for (int ix = 0; ix < ntx; ++ix)
-        for (int iz = L; iz < pml; ++iz) {
+    OFWI_XZ(ix, 0, ntx, iz, L, pml) {
             const long ip = (long)iz * ntx + ix;
             const REAL wt = w[iz - L];
             p2[ip] = wt * SUF(abc_h)(p0, p1, ip, nx_l, t11[ip], t12[ip], t13[ip]) + (1.0 - wt) * p2[ip];
         }
 #pragma omp parallel for collapse(2)
-    for (int ix = 0; ix < ntx; ++ix)
-        for (int iz = ntz - pml; iz < ntz - L; ++iz) {
+    OFWI_XZ(ix, 0, ntx, iz, ntz - pml, ntz - L) {
             const long ip = (long)iz * ntx + ix;
             const REAL wt = w[ntz - L - iz - 1];
             p2[ip] = wt * SUF(abc_h)(p0, p1, ip, -nx_l, t11[ip], t12[ip], t13[ip]) + (1.0 - wt) * p2[ip];
@@ -137,8 +132,7 @@ static void SUF(rotate)(SUF(ofwi_ctx) *c)
     REAL *p0 = c->p0, *p1 = c->p1;
     const REAL *p2 = c->p2;
 #pragma omp parallel for collapse(2)
-    for (int ix = 0; ix < ntx; ++ix)
-        for (int iz = 0; iz < ntz; ++iz) {
+    OFWI_XZ(ix, 0, ntx, iz, 0, ntz) {
             const long ip = (long)iz * ntx + ix;
             p0[ip] = p1[ip];
             p1[ip] = p2[ip];
@@ -165,8 +159,7 @@ static void SUF(forward_io)(SUF(ofwi_ctx) *c, int it, int is, int run_fwi)
         REAL *dst = c->psave + (long)(it / c->k) * c->ntp;
         const REAL *p0 = c->p0, *p1 = c->p1, *p2 = c->p2;
 #pragma omp parallel for collapse(2)
-        for (int ix = 0; ix < ntx; ++ix)
-            for (int iz = 0; iz < ntz; ++iz) {
+        OFWI_XZ(ix, 0, ntx, iz, 0, ntz) {
                 const long ip = (long)iz * ntx + ix;
                 if (run_fwi)
                     dst[ip] = (p2[ip] - 2 * p1[ip] + p0[ip]) / dt / dt;
@@ -194,8 +187,7 @@ static void SUF(backward_io)(SUF(ofwi_ctx) *c, int it, REAL *grad, int run_fwi)
         const REAL *p2 = c->p2, *v = c->velp;
         const int nxi = ntx - 2 * pml;
 #pragma omp parallel for collapse(2)
-        for (int ix = pml; ix < ntx - pml; ++ix)
-            for (int iz = pml; iz < ntz - pml; ++iz) {
+        OFWI_XZ(ix, pml, ntx - pml, iz, pml, ntz - pml) {
                 const long ip = (long)iz * ntx + ix;
                 const long gi = (long)(iz - pml) * nxi + (ix - pml);
                 if (run_fwi)
@@ -261,8 +253,7 @@ static int SUF(ctx_init)(SUF(ofwi_ctx) *c, const ofwi_params *p, const REAL *vel
     const REAL alpha = 1.0, dt = c->dt, dx = c->dx;
     const int ntx = c->ntx, ntz = c->ntz;
 #pragma omp parallel for collapse(2)
-    for (int ix = 0; ix < ntx; ++ix)
-        for (int iz = 0; iz < ntz; ++iz) {
+    OFWI_XZ(ix, 0, ntx, iz, 0, ntz) {
             const long ip = (long)iz * ntx + ix;
             const REAL s = alpha * c->velp[ip] * dt / dx;
             c->t11[ip] = (2 - s) * (1 - s) / 2;

@@ -12,8 +12,9 @@ model, 30 shots, 300 receivers, 4000 steps, checkpoints every 100 steps.
   about 10 % (relative L2 over the 10 directions, after the best positive
   scale) from the discrete misfit's derivative: measured 0.103 with cosine
   0.9947 and scale 1146, unchanged between eps = 1 and 4 m/s (so not
-  rounding).  The bar is cosine > 0.98, relative L2 < 0.15, scale within a
-  factor 2 of 1/dt; the exact bar (1e-4) is against the oracle, below.
+  rounding).  The bar is cosine > 0.98, relative L2 < 0.15, scale in
+  [0.7/dt, 1.4/dt] (a window a factor 2 either way leaves); the exact bar
+  (1e-4) is against the oracle, below.
 * Against the reference gradient: the CPU oracle on the first and last shot
   of the same geometry (the gradient is a sum over shots, so two shots at
   the full grid and length test the same per-shot work).
@@ -79,7 +80,7 @@ def test_autograd_gradient_against_finite_differences(c2):
     dt = float(np.float32(p.dt))
     assert np.isfinite(loss.item()) and loss.item() > 0
     assert cos > 0.98 and rel < 0.15
-    assert 0.5 / dt < alpha < 2.0 / dt
+    assert 0.7 / dt < alpha < 1.4 / dt
 
 
 def test_first_and_last_shot_against_oracle(c2, oracle_mod):

@@ -1,5 +1,6 @@
 """The headline workload at full size: 16 samples x 20 shots on the
-200 x 400 grid, 200 receivers, 2000 steps.
+200 x 400 grid, 200 receivers, 2000 steps; observed data with additive
+Gaussian noise at 1 % of their RMS.
 
 * Checkpointed (every 100 steps) and full-store batches give the same bits
   for all 16 samples.
@@ -34,8 +35,9 @@ def c1():
     p = synth.workload_params("c1")
     vt = synth.true_model("c1", seed=1)
     samples = synth.gaussian_samples(synth.smooth(vt, 8.0), 50.0, 16, seed=101).astype(np.float32)
-    obs = solver.forward_batch(p, dev(vt).view(1, -1))[0].contiguous()
-    print(f"c1: {p.nz}x{p.nx}, {p.ns} shots, {p.nt} steps, 16 samples", flush=True)
+    clean = host(solver.forward_batch(p, dev(vt).view(1, -1))[0])
+    obs = dev(synth.add_noise(clean, 0.01, seed=201))   # 1 % RMS noise, as the workload specifies
+    print(f"c1: {p.nz}x{p.nx}, {p.ns} shots, {p.nt} steps, 16 samples, 1 % noise", flush=True)
     return p, vt, samples, obs
 
 

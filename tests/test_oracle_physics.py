@@ -106,13 +106,17 @@ def test_gradient_against_finite_differences(oracle_mod):
     perturbation of an edge cell moves a whole line of band cells next to the
     surface sources and receivers.  So the directions vanish on the model's
     edge, and after the best positive scale the directional derivatives
-    agree within 5 %, their cosine exceeds 0.99, and the scale is within a
-    factor 2 of 1/dt."""
+    agree within 5 %, their cosine exceeds 0.99, and the scale lies in
+    [0.7/dt, 1.4/dt] (0.77/dt measured), a window that a factor 2 either way
+    leaves.  The imaging interval does not change the scale: at
+    nt_interval = 2 it is the k = 1 scale within 1 % (a dropped or doubled
+    factor k, fdtd2D_modelling.c:965, would halve or double it)."""
     p = synth.workload_params("tiny", nt_interval=1)
     vt = synth.gradient_anomaly_model(p.nz, p.nx, seed=6)
     vs = synth.smooth(vt, 3.0)
     obs = oracle_mod.forward(p, vt, dtype=np.float64)
     _, g = oracle_mod.fwi(p, vs, obs, dtype=np.float64)
+    _, g2 = oracle_mod.fwi(p.replace(nt_interval=2), vs, obs, dtype=np.float64)
     grad_misfit = -g
 
     def misfit(v):
@@ -120,18 +124,21 @@ def test_gradient_against_finite_differences(oracle_mod):
         return 0.5 * np.sum((s - obs) ** 2)
 
     rng = np.random.default_rng(11)
-    fd, an = [], []
+    fd, an, an2 = [], [], []
     for _ in range(4):
         d = synth.smooth(rng.standard_normal((p.nz, p.nx)), 2.0) * synth.edge_taper(p.nz, p.nx, 3)
         d /= np.abs(d).max()
         eps = 1.0  # m/s
         fd.append((misfit(vs + eps * d) - misfit(vs - eps * d)) / (2 * eps))
         an.append(float(np.dot(grad_misfit, d.reshape(-1))))
-    fd, an = np.array(fd), np.array(an)
+        an2.append(float(np.dot(-g2, d.reshape(-1))))
+    fd, an, an2 = np.array(fd), np.array(an), np.array(an2)
     cos = float(np.dot(fd, an) / (np.linalg.norm(fd) * np.linalg.norm(an)))
     alpha = float(np.dot(fd, an) / np.dot(an, an))
+    alpha2 = float(np.dot(fd, an2) / np.dot(an2, an2))
     rel = float(np.linalg.norm(fd - alpha * an) / np.linalg.norm(fd))
-    print(f"FD check: cos={cos:.5f} scale={alpha:.1f} rel-after-scale={rel:.3e}")
+    print(f"FD check: cos={cos:.5f} scale={alpha:.1f} (k=2: {alpha2:.1f}) rel-after-scale={rel:.3e}")
     dt = float(np.float32(p.dt))
     assert cos > 0.99 and rel < 0.05
-    assert 0.5 / dt < alpha < 2.0 / dt
+    assert 0.7 / dt < alpha < 1.4 / dt
+    assert abs(alpha2 / alpha - 1.0) < 0.01
