@@ -58,10 +58,11 @@ def parse():
     ap.add_argument("--serve-rate", type=float, default=50.0, help="Poisson arrival rate (requests/s)")
     ap.add_argument("--serve-low-rate", type=float, default=10.0, help="sub-saturation run rate (0 = skip)")
     ap.add_argument("--serve-low-requests", type=int, default=60)
-    ap.add_argument("--serve-c3", type=int, default=0,
-                    help="also stream this many requests of the full C3 mix (grids up to 400x800, up to 64 "
-                         "samples x 40 shots x 3000 steps; minutes of device time)")
-    ap.add_argument("--cpu-shots", type=int, default=20)
+    ap.add_argument("--serve-c3", type=int, default=6,
+                    help="stream this many requests of the serving workload's mix (grids up to 400x800, up to "
+                         "64 samples x 40 shots x 3000 steps) at --serve-rate; 0 = skip")
+    ap.add_argument("--cpu-shots", type=int, default=5, help="shots of one c1 sample per CPU-baseline run")
+    ap.add_argument("--cpu-runs", type=int, default=3, help="timed CPU-baseline runs (median) after a warm-up")
     ap.add_argument("--no-vi", action="store_true", help="skip the end-to-end inversion leg")
     ap.add_argument("--vi-iterations", type=int, default=2)
     ap.add_argument("--no-psm", action="store_true", help="skip the pseudo-spectral (laplace_slover=1) leg")
@@ -472,7 +473,10 @@ def main():
                      "launches_per_step": launches, "avg_launch_us": round(avg_launch_s * 1e6, 3)},
     }
     if tune is not None:
-        out["autotune"] = {"us_per_time_step": tune["us_per_step"], "candidates": len(tune["timings"])}
+        out["autotune"] = {"schedule": "forward with store + adjoint with imaging, 64 steps",
+                           "chosen": tune.get("applied", f"{tune['variant']},{tune['grid']},{tune['tb']}"),
+                           "us_per_time_step": tune["us_per_step"], "candidates": len(tune["timings"]),
+                           "us_per_launch_full_evaluation": round(avg_launch_s * 1e6, 3)}
     del ws
     # What a plain streaming kernel with the step's read:write mix reaches on
     # this GPU (torch's addcmul: 3 reads, 1 write), for scale.
@@ -514,6 +518,7 @@ def main():
         out["serving_c3"] = {"mix": "c3 (grids 100x200..400x800, 4-64 samples, 5-40 shots, 1000-3000 steps)",
                              **serve_stream(args, args.serve_rate, args.serve_c3, rng_seed + 700, dev, mix="c3")}
         log(f"c3 stream of {args.serve_c3} requests done")
+        torch.cuda.empty_cache()
 
     if not args.no_vi:
         out["vi"] = vi_leg(args, dev)
@@ -529,15 +534,22 @@ def main():
         oracle.set_threads(ncores)
         shots = max(1, min(args.cpu_shots, p.ns))
         v0 = samples[0].astype(np.float32)
-        t0 = time.perf_counter()
-        oracle.fwi(p, v0, obs_np, shots=(0, shots))
-        tc = time.perf_counter() - t0
-        log(f"cpu oracle: {tc:.2f} s")
+        oracle.fwi(p, v0, obs_np, shots=(0, 1))   # warm-up
+        runs = []
+        for _ in range(max(1, args.cpu_runs)):
+            t0 = time.perf_counter()
+            oracle.fwi(p, v0, obs_np, shots=(0, shots))
+            runs.append(time.perf_counter() - t0)
+        tc = float(np.median(runs))
+        log(f"cpu oracle: {shots} shots, median {tc:.2f} s of {len(runs)} runs")
         per_sample = tc * p.ns / shots
         out["cpu_baseline"] = {"value": round(1.0 / per_sample, 6), "unit": "samples/s", "cores": ncores,
-                               "kind": "port",
+                               "kind": "port", "cpu_model": _cpu_model(),
+                               "runs_s": [round(t, 3) for t in runs],
                                "sample": f"{shots} of {p.ns} shots of one c1 sample through the C oracle "
-                                         f"(OpenMP, reference loop order): {tc:.2f} s, scaled to a sample"}
+                                         f"(OpenMP on {ncores} threads, reference loop order, full per-call "
+                                         f"set-up); median of {len(runs)} runs after a one-shot warm-up: "
+                                         f"{tc:.2f} s, scaled to the sample's {p.ns} shots"}
         out["c0_baselines"] = c0_baselines(args, ncores, dev)
     print(json.dumps(out), flush=True)
 

@@ -5,10 +5,12 @@ ABI): the strip-segment height (tile variant: rows per wavefront work item),
 the grid (0 = one wavefront per work item) and the time steps per launch
 (1 or 2).  Which combination is fastest depends on the grid size, the number
 of (sample, shot) units and the device, so this module measures it: every
-candidate runs the forward pass of the real geometry for a shortened number
-of steps on synthetic velocities, timed with HIP events, and the fastest one
-is applied and remembered in a JSON cache keyed by device and workload
-shape.  Every candidate computes bit-identical results (they only change
+candidate runs a whole misfit+gradient evaluation of the real geometry --
+forward pass with the wavefield store every nt_interval steps, then the
+adjoint with the imaging condition on the same steps, the mix of launch
+kinds a real evaluation has -- shortened to a few dozen time steps, on
+synthetic velocities, timed with HIP events; the fastest one is applied and
+remembered in a JSON cache keyed by device and workload shape.  Every candidate computes bit-identical results (they only change
 which wavefront computes which points), so tuning never changes outputs.
 """
 from __future__ import annotations
@@ -78,10 +80,16 @@ def apply(cand: Candidate, graph_steps: int = 32) -> None:
 
 
 def measure(params: FWIParams, nsample: int, cands: Sequence[Candidate], steps: int = 64, reps: int = 2,
-            device=None, timer: Optional[Callable[[Candidate], float]] = None) -> Dict[Candidate, float]:
-    """Microseconds per time step of the forward pass for each candidate.
+            device=None, timer: Optional[Callable[[Candidate], float]] = None,
+            schedule: str = "evaluation") -> Dict[Candidate, float]:
+    """Microseconds per time step (one forward or adjoint step of every
+    unit) for each candidate.  schedule "evaluation": forward with store and
+    adjoint with imaging over `steps` steps (a real evaluation's launch mix);
+    "forward": forward modelling only (no store).
 
     `timer` replaces the GPU measurement (tests of the selection logic)."""
+    if schedule not in ("evaluation", "forward"):
+        raise ValueError(f"schedule {schedule!r}: 'evaluation' or 'forward'")
     if timer is not None:
         return {c: float(timer(c)) for c in cands}
     import numpy as np
@@ -95,21 +103,32 @@ def measure(params: FWIParams, nsample: int, cands: Sequence[Candidate], steps: 
     base = synth.gradient_anomaly_model(params.nz, params.nx, params.dx, params.dz, seed=0)
     vel = synth.gaussian_samples(synth.smooth(base, 4.0), 20.0, nsample, seed=1).astype(np.float32)
     vel_t = torch.from_numpy(vel.reshape(nsample, -1)).to(dev)
+    if schedule == "evaluation":
+        obs = solver.forward_batch(short, vel_t[:1])[0].contiguous()
+        passes = 2
+
+        def run():
+            solver.fwi_batch(short, vel_t, obs)
+    else:
+        passes = 1
+
+        def run():
+            solver.forward_batch(short, vel_t)
     saved = (_lib.get_tuning(), _lib.get_time_blocking())
     out: Dict[Candidate, float] = {}
     try:
         for cand in cands:
             apply(cand, saved[0][2])
-            solver.forward_batch(short, vel_t)  # capture the graph, warm the caches
+            run()  # capture the graphs, warm the caches
             torch.cuda.synchronize(dev)
             e0 = torch.cuda.Event(enable_timing=True)
             e1 = torch.cuda.Event(enable_timing=True)
             e0.record()
             for _ in range(reps):
-                solver.forward_batch(short, vel_t)
+                run()
             e1.record()
             torch.cuda.synchronize(dev)
-            out[cand] = e0.elapsed_time(e1) * 1e3 / (reps * short.nt)
+            out[cand] = e0.elapsed_time(e1) * 1e3 / (reps * passes * short.nt)
     finally:
         (v, g, s), tb = saved
         _lib.set_tuning(v, g, s)
@@ -119,7 +138,8 @@ def measure(params: FWIParams, nsample: int, cands: Sequence[Candidate], steps: 
 
 def autotune(params: FWIParams, nsample: int, cands: Optional[Sequence[Candidate]] = None, steps: int = 64,
              cache_path: Optional[str] = DEFAULT_CACHE, force: bool = False, device=None,
-             device_name: Optional[str] = None, timer: Optional[Callable[[Candidate], float]] = None) -> dict:
+             device_name: Optional[str] = None, timer: Optional[Callable[[Candidate], float]] = None,
+             schedule: str = "evaluation") -> dict:
     """Pick and apply the fastest launch shape for (params, nsample).
 
     Returns {"variant", "grid", "tb", "us_per_step", "cached", "timings"}.
@@ -131,7 +151,7 @@ def autotune(params: FWIParams, nsample: int, cands: Optional[Sequence[Candidate
         else:
             import torch
             device_name = torch.cuda.get_device_name(0 if device is None else device)
-    key = shape_key(params, nsample, device_name)
+    key = shape_key(params, nsample, device_name) + f"|{schedule}"
     if cache_path and not force:
         hit = load_cache(cache_path).get(key)
         if hit and all(k in hit for k in ("variant", "grid", "tb")):
@@ -140,11 +160,12 @@ def autotune(params: FWIParams, nsample: int, cands: Optional[Sequence[Candidate
                 apply(cand)
             return {**hit, "cached": True}
     cands = list(cands) if cands is not None else candidates()
-    timings = measure(params, nsample, cands, steps=steps, device=device, timer=timer)
+    timings = measure(params, nsample, cands, steps=steps, device=device, timer=timer, schedule=schedule)
     best = select(timings)
     if timer is None:
         apply(best)
     entry = {"variant": best[0], "grid": best[1], "tb": best[2], "us_per_step": round(timings[best], 3),
+             "schedule": schedule, "steps": int(steps),
              "timings": {f"{v},{g},{t}": round(us, 3) for (v, g, t), us in timings.items()}}
     if cache_path:
         store_cache(key, entry, cache_path)

@@ -103,16 +103,22 @@ int validate(const gpvi_fwi2d_params* p)
 // ---------------------------------------------------------------------------
 // Constant-divisor division (see cdiv_fast in fwi2d_step.h): for a divisor
 // b in [2^-16, 2^16] with y = RN(1/b), q = RN(q0 + RN(a - b*q0)*y), q0 = RN(a*y),
-// equals RN(a/b) for a == 0 or 2^-80 <= |a| < 2^80 (Markstein; no
-// intermediate underflow or overflow there).  Division by b is invariant
-// under a -> 2a in that range, so checking every significand of one binade
-// checks the whole range; the result is cached per divisor.
+// equals RN(a/b) for a == 0 or 2^-80 <= |a| < 2^80 (Markstein).  The kernels
+// also divide that quotient by b again ((a/b)/b), whose dividend lies in
+// [2^-96, 2^96].  Over a dividend range of [2^-96, 2^96] no intermediate
+// under- or overflows: q0 lies in [2^-112, 2^112] (normal), and the exact
+// remainder a - b*q0 is a multiple of 2^(e(q0)+e(b)-46) >= 2^-143, so the
+// FMA forms it exactly even when it is subnormal.  Every rounding decision
+// is then the one of the dividend scaled by a power of two into [1, 2): one
+// binade, checked in full, settles the identity for the whole range; the
+// binades at both ends of the range are sampled as well.  The result is
+// cached per divisor.
 
-__attribute__((target("fma"))) static bool markstein_binade_fma(float b, float y)
+__attribute__((target("fma"))) static bool markstein_binade_fma(float b, float y, int e, uint32_t stride)
 {
-    for (uint32_t m = 0; m < (1u << 23); ++m) {
+    for (uint32_t m = 0; m < (1u << 23); m += stride) {
         float a;
-        const uint32_t bits = 0x3f800000u | m;
+        const uint32_t bits = (uint32_t)(127 + e) << 23 | m;
         std::memcpy(&a, &bits, 4);
         volatile float vb = b;
         const float q = a / vb;
@@ -124,11 +130,11 @@ __attribute__((target("fma"))) static bool markstein_binade_fma(float b, float y
     return true;
 }
 
-static bool markstein_binade_soft(float b, float y)
+static bool markstein_binade_soft(float b, float y, int e, uint32_t stride)
 {
-    for (uint32_t m = 0; m < (1u << 23); m += 97) {
+    for (uint32_t m = 0; m < (1u << 23); m += stride) {
         float a;
-        const uint32_t bits = 0x3f800000u | m;
+        const uint32_t bits = (uint32_t)(127 + e) << 23 | m;
         std::memcpy(&a, &bits, 4);
         volatile float vb = b;
         const float q = a / vb;
@@ -137,6 +143,18 @@ static bool markstein_binade_soft(float b, float y)
         const float q1 = std::fma(r, y, q0);
         if (q != q1) return false;
     }
+    return true;
+}
+
+static bool markstein_ok(float b, float y)
+{
+    const bool hw = __builtin_cpu_supports("fma");
+    auto check = [&](int e, uint32_t stride) {
+        return hw ? markstein_binade_fma(b, y, e, stride) : markstein_binade_soft(b, y, e, stride * 97);
+    };
+    if (!check(0, 1)) return false;                           // [1, 2): every significand
+    for (int e : {-96, -80, 79, 95})                          // the ends of both divisions' ranges
+        if (!check(e, 61)) return false;
     return true;
 }
 
@@ -154,8 +172,7 @@ bool fastdiv_ok(float b, float* recip)
     std::lock_guard<std::mutex> lk(g_div_mu);
     auto it = g_div_ok.find(key);
     if (it != g_div_ok.end()) return it->second;
-    const bool ok = __builtin_cpu_supports("fma") ? markstein_binade_fma(b, *recip)
-                                                   : markstein_binade_soft(b, *recip);
+    const bool ok = markstein_ok(b, *recip);
     g_div_ok[key] = ok;
     return ok;
 }

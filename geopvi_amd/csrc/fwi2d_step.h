@@ -39,9 +39,12 @@ __device__ __forceinline__ __attribute__((address_space(1))) T* G(T* p)
 // Division by a per-job constant b with y = RN(1/b): q0 = a*y, r = a - b*q0
 // (exact through the FMA), q = RN(q0 + r*y) is the correctly rounded a/b
 // (Markstein's theorem) whenever no intermediate underflows or overflows.
-// That holds for 2^-16 <= |b| <= 2^16 and a == 0 or 2^-80 <= |a| < 2^80;
-// other lanes take the IEEE division.  The host checks the divisor range and
-// verifies the identity over a full binade before it sets UnitDesc::fastdiv.
+// That holds for 2^-16 <= |b| <= 2^16 and a == 0 or 2^-96 <= |a| <= 2^96.
+// The range test admits a first dividend in [2^-80, 2^80); its quotient, the
+// dividend of the second division of (a/b)/b, then lies in [2^-96, 2^96] (or
+// is 0), so one test of a covers both divisions.  Other lanes take the IEEE
+// division.  The host checks the divisor range and verifies the identity over
+// a full binade and at the ends of the range before it sets UnitDesc::fastdiv.
 // `hi` is 2^80 when the job may use the identity and 0 when it may not (then
 // no value is in range and every lane divides).
 __device__ __forceinline__ bool cdiv_in_range(float a, float hi)

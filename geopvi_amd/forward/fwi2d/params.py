@@ -42,6 +42,18 @@ _FLOAT_FIELDS = (
 )
 
 
+def _fd_weights(lc: int):
+    """Central second-derivative weights of half-width lc = Lc + 1 (the
+    exact rationals of fdtd2D_modelling.c:1272-1326)."""
+    table = {4: (-49 / 18, 3 / 2, -3 / 20, 1 / 90),
+             5: (-205 / 72, 8 / 5, -1 / 5, 8 / 315, -1 / 560),
+             6: (-5269 / 1800, 5 / 3, -5 / 21, 5 / 126, -5 / 1008, 1 / 3150),
+             7: (-5369 / 1800, 12 / 7, -15 / 56, 10 / 189, -1 / 112, 2 / 1925, -1 / 16632)}
+    if lc not in table:
+        raise ValueError(f"Lc={lc - 1}: supported 3..6")
+    return table[lc]
+
+
 class CParams(ctypes.Structure):
     """C layout of the parameter block (``gpvi_fwi2d_params`` in
     include/geopvi_fwi2d.h): 15 int32 then 4 float32."""
@@ -194,6 +206,24 @@ class FWIParams:
         for name, key, _ in _FLOAT_FIELDS:
             kw[name] = config.getfloat("FWI", key)
         return cls(**kw)
+
+    # --- stability of the time stepping -----------------------------------
+    def max_stable_velocity(self) -> float:
+        """Largest velocity the explicit scheme is stable for.
+
+        The update u+ = 2u - u- + (v dt)^2 L u (fdtd2D_modelling.c:776) is
+        stable when (v dt)^2 * lambda_max(-L) <= 4.  The central
+        second-derivative stencil with weights fd_m (fdtd2D_modelling.c:
+        1272-1326) has its largest symbol at the Nyquist wavenumber:
+        s = -(fd_0 + 2 sum_m fd_m (-1)^m), so lambda_max(-L) =
+        s (1/dx^2 + 1/dz^2); the pseudo-spectral Laplacian (laplace_slover=1)
+        has s = pi^2.  For half-order 3 at dx = dz this is v dt/dx <= 0.575."""
+        if self.laplace_slover == 1:
+            s = math.pi ** 2
+        else:
+            fd = _fd_weights(self.lc)
+            s = -(fd[0] + 2.0 * sum(w * (-1) ** m for m, w in enumerate(fd) if m > 0))
+        return 2.0 / (self.dt * math.sqrt(s * (1.0 / self.dx ** 2 + 1.0 / self.dz ** 2)))
 
     # --- geometry (fdtd2D_modelling.c:118-152) -------------------------------
     def source_ix(self):

@@ -25,6 +25,8 @@ from __future__ import annotations
 
 import bisect
 import collections
+import json
+import math
 import threading
 import time
 from concurrent.futures import Future
@@ -126,6 +128,22 @@ def choose_store(full_bytes: int, ckpt_bytes: int, room: int, budget: int, polic
     return fits and full_bytes * full_share <= budget
 
 
+def check_models(params: FWIParams, vel: torch.Tensor) -> None:
+    """Refuse models the solver cannot run meaningfully: any non-finite
+    value (the reference stops on NaN, pyacofwi2D.pyx:16-18 and
+    geopvi/forward/fwi2d/posterior.py:159-160) or a velocity above the
+    explicit scheme's stability limit (FWIParams.max_stable_velocity), where
+    the fields grow without bound.  One reduction and one small copy on the
+    caller's stream."""
+    vmax = float(vel.abs().amax())   # NaN and Inf propagate through amax
+    if not math.isfinite(vmax):
+        raise ValueError("NaN or Inf in the velocity models")
+    limit = params.max_stable_velocity()
+    if vmax > limit:
+        raise ValueError(f"velocity {vmax:.1f} m/s exceeds the stability limit {limit:.1f} m/s of dt={params.dt}, "
+                         f"dx={params.dx}, dz={params.dz} (v*dt/dx = {vmax * params.dt / params.dx:.3f})")
+
+
 @dataclass
 class _Split:
     """A request whose samples run as several engine jobs (chunks of samples
@@ -172,11 +190,17 @@ class FWIServer:
     time steps advanced between admission/completion checks (rounded to
     whole graph segments by the engine).  store_policy: how a request
     without an explicit ckpt_interval keeps its forward wavefield (see
-    choose_store).
+    choose_store).  validate_models: refuse, at submit(), models with
+    non-finite values or velocities above the scheme's stability limit
+    (check_models).  log_path: append one JSON line per finished or failed
+    request (ids, times, latency, sizes, store mode).  A job whose misfit
+    comes out non-finite (for instance from NaN or Inf in its observed data)
+    fails alone with FloatingPointError; the jobs beside it are unaffected.
     """
 
     def __init__(self, device: int = 0, mem_budget: Optional[int] = None, steps_per_tick: int = 64,
-                 max_units: int = 16384, max_tiles: int = 1 << 21, store_policy: str = "auto"):
+                 max_units: int = 16384, max_tiles: int = 1 << 21, store_policy: str = "auto",
+                 validate_models: bool = True, log_path: Optional[str] = None):
         choose_store(0, 0, 0, 1, store_policy)  # validates the name
         self.device = torch.device("cuda", device)
         torch.cuda.set_device(self.device)
@@ -188,7 +212,11 @@ class FWIServer:
         self.arena = Arena(mem_budget, self.device)
         self.mem_budget = self.arena.size
         self.store_policy = store_policy
+        self.validate_models = bool(validate_models)
         self.steps_per_tick = int(steps_per_tick)
+        self._log = open(log_path, "a", buffering=1) if log_path else None
+        self._next_req = 0
+        self._side = torch.cuda.Stream(self.device)  # reads finished outputs without waiting for queued steps
         self.engine = Engine(self.stream, max_units=max_units, max_tiles=max_tiles)
         self._queue: collections.deque = collections.deque()
         self._inflight: dict = {}
@@ -213,6 +241,8 @@ class FWIServer:
         if not isinstance(vel, torch.Tensor) or not vel.is_cuda:
             vel = torch.as_tensor(np.asarray(vel, np.float32)).to(self.device)
         vel = vel.to(torch.float32).reshape(-1, params.ni).contiguous()
+        if self.validate_models:
+            check_models(params, vel)
         obs = record_obs if isinstance(record_obs, torch.Tensor) else torch.as_tensor(record_obs)
         obs = obs.to(self.device, torch.float32).reshape(-1).contiguous()
         if obs.numel() != params.record_len:
@@ -234,10 +264,13 @@ class FWIServer:
                                   f"workspace; the arena has {self.arena.size / 1e9:.1f} GB")
         fut: Future = Future()
         reqs = []
+        with self._cv:
+            rid = self._next_req
+            self._next_req += 1
         if chunk >= n:
             ev = torch.cuda.Event()
             ev.record()  # inputs written on the caller's stream are ready here
-            reqs.append(_Request(params, vel, obs, int(run_fwi), ckpt, need, ev, fut, t_sub))
+            reqs.append(_Request(params, vel, obs, int(run_fwi), ckpt, need, ev, fut, t_sub, extra={"id": rid}))
         else:
             split = _Split(n, (n + chunk - 1) // chunk, t_sub)
             ev = torch.cuda.Event()
@@ -246,7 +279,7 @@ class FWIServer:
                 hi = min(n, lo + chunk)
                 reqs.append(_Request(params, vel[lo:hi], obs, int(run_fwi), ckpt,
                                      job_workspace_bytes(params, hi - lo, ck_plan), ev, fut, t_sub,
-                                     split=split, rows=(lo, hi)))
+                                     extra={"id": rid}, split=split, rows=(lo, hi)))
         with self._cv:
             if self._error is not None:
                 raise RuntimeError("server stopped after an error") from self._error
@@ -273,6 +306,9 @@ class FWIServer:
         if wait:
             self._thread.join()
         self.engine.close()
+        if self._log is not None:
+            self._log.close()
+            self._log = None
         if wait:
             self.arena = None
             torch.cuda.empty_cache()  # the arena's memory goes back to the device
@@ -380,6 +416,27 @@ class FWIServer:
             self._queue = collections.deque(r for r in self._queue if r.split is not req.split)
         if not req.future.done():
             req.future.set_exception(exc)
+            self._write_log(req, None, f"failed: {exc}")
+
+    def _nonfinite_rows(self, misfit: torch.Tensor):
+        """Rows of a finished job's misfit that are NaN or Inf.  The job's
+        kernels are complete, so the copy runs on a side stream and does not
+        wait for the steps queued behind them."""
+        with torch.cuda.stream(self._side):
+            m = misfit.to("cpu")
+        return np.flatnonzero(~np.isfinite(m.numpy()))
+
+    def _write_log(self, req, res: Optional[FWIResult], status: str):
+        if self._log is None:
+            return
+        p = req.params
+        rec = {"id": req.extra.get("id"), "status": status, "grid": [p.nz, p.nx], "ns": p.ns, "nt": p.nt,
+               "nsample": req.split.nsample if req.split is not None else int(req.vel.shape[0]),
+               "submitted": req.submitted}
+        if res is not None:
+            rec.update(admitted=res.admitted, finished=res.finished, latency_s=res.latency,
+                       ckpt_interval=res.extra.get("ckpt_interval"), chunks=res.extra.get("chunks", 1))
+        self._log.write(json.dumps(rec) + "\n")
 
     def _reap(self):
         done = [jid for jid in self._inflight if self.engine.status(jid) == 1]
@@ -392,21 +449,33 @@ class FWIServer:
             # the job's last kernels are complete (status 1), so its arena
             # block can be handed to the next job
             self.arena.release(req.extra["arena_offset"], req.need)
+            bad = self._nonfinite_rows(job.misfit)
             sp = req.split
+            if len(bad) and not req.future.done():
+                lo = req.rows[0] if sp is not None else 0
+                exc = FloatingPointError(f"non-finite misfit for sample(s) {(bad + lo).tolist()}: unstable model "
+                                         "or NaN/Inf in the observed data")
+                req.future.set_exception(exc)
+                self._write_log(req, None, f"failed: {exc}")
+                if sp is not None:  # drop the request's chunks still queued
+                    with self._cv:
+                        self._queue = collections.deque(r for r in self._queue if r.split is not sp)
             if sp is None:
+                if req.future.done():
+                    continue
                 res = FWIResult(job.misfit, job.grad, job.syn, req.submitted, req.admitted, time.perf_counter())
                 res.extra["ckpt_interval"] = req.ckpt
             else:
                 sp.remaining -= 1
                 sp.ckpts.append(req.ckpt)
-                if sp.remaining:
+                if sp.remaining or req.future.done():
                     continue
                 res = FWIResult(sp.misfit, sp.grad, sp.syn, sp.submitted, sp.admitted, time.perf_counter())
                 res.extra["ckpt_interval"] = sp.ckpts[0] if len(set(sp.ckpts)) == 1 else list(sp.ckpts)
                 res.extra["chunks"] = len(sp.ckpts)
             self.completed.append(res)
-            if not req.future.done():  # a failed split's remaining chunks resolve nothing
-                req.future.set_result(res)
+            req.future.set_result(res)
+            self._write_log(req, res, "ok")
 
     def _run(self):
         torch.cuda.set_device(self.device)
@@ -421,7 +490,12 @@ class FWIServer:
                         self._admit()
                     if self._inflight:
                         self.engine.advance(self.steps_per_tick)
+                        n_before = len(self._inflight)
                         self._reap()
+                        if len(self._inflight) == n_before and self.engine.pending_steps() == 0:
+                            # every active job has all its steps queued: wait
+                            # for the device instead of polling the events
+                            self.engine.sync()
         except BaseException as exc:  # surface the failure to every waiter
             self._error = exc
             for req in list(self._queue) + list(self._inflight.values()):

@@ -31,7 +31,7 @@ def test_autotune_with_timer_and_cache(tmp_path):
     assert (r1["variant"], r1["grid"], r1["tb"]) == (0, 0, 2)
     assert not r1["cached"] and len(calls) == len(cost)
     data = json.load(open(path))
-    key = autotune.shape_key(p, 4, "test")
+    key = autotune.shape_key(p, 4, "test") + "|evaluation"
     assert data[key]["variant"] == 0 and data[key]["tb"] == 2
     # second call: cache hit, nothing measured
     r2 = autotune.autotune(p, 4, cache_path=path, timer=timer)

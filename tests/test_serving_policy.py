@@ -63,3 +63,35 @@ def test_arena_random_alloc_release_never_overlaps():
         spans = sorted((o, o + Arena._round(n)) for o, n in live.items())
         assert all(e0 <= s1 for (_, e0), (s1, _) in zip(spans, spans[1:]))
         assert a.free_bytes() + sum(Arena._round(n) for n in live.values()) == a.size
+
+
+def test_check_models_refuses_nan_inf_and_unstable_velocities():
+    from geopvi_amd import synth
+    from geopvi_amd.serving import check_models
+    p = synth.workload_params("tiny")
+    limit = p.max_stable_velocity()
+    assert abs(limit * p.dt / p.dx - 0.5752) < 1e-3          # half-order 3, dx = dz
+    ok = torch.full((2, p.ni), 0.99 * limit)
+    check_models(p, ok)
+    for bad_value in (float("nan"), float("inf"), -float("inf"), 1.01 * limit, -1.01 * limit):
+        bad = ok.clone()
+        bad[1, 17] = bad_value
+        with pytest.raises(ValueError):
+            check_models(p, bad)
+
+
+@pytest.mark.parametrize("Lc", [3, 6])
+def test_stability_limit_against_the_oracle(oracle_mod, Lc):
+    """The stability limit is where the scheme itself turns unstable: a
+    homogeneous model 2 % below it stays bounded over 1500 steps, 3 % above
+    it grows by orders of magnitude (the Nyquist mode, seeded by rounding)."""
+    from geopvi_amd import synth
+    p = synth.workload_params("tiny", Lc=Lc, nt=1500, f0=10.0)
+    limit = p.max_stable_velocity()
+    below = oracle_mod.forward(p, np.full((p.nz, p.nx), 0.98 * limit), dtype=np.float64)
+    above = oracle_mod.forward(p, np.full((p.nz, p.nx), 1.03 * limit), dtype=np.float64)
+    amp_below = np.abs(below).max()
+    amp_above = np.abs(above[np.isfinite(above)]).max() if np.isfinite(above).any() else np.inf
+    print(f"Lc={Lc}: max |record| below {amp_below:.3e}, above {amp_above:.3e}")
+    assert np.isfinite(below).all() and amp_below < 1e3
+    assert amp_above > 1e6 * amp_below or not np.isfinite(above).all()
