@@ -77,8 +77,9 @@ int validate(const gpvi_fwi2d_params* p)
     if (!psm && (p->Lc < 3 || p->Lc > 6)) return fail(GPVI_ERR_UNSUPPORTED, "Lc=%d: supported 3..6", p->Lc);
     if (p->nx < 1 || p->nz < 1 || p->ns < 1 || p->nt < 1 || p->nr < 1 || p->nt_interval < 1 || p->pml0 < 0)
         return fail(GPVI_ERR_INVALID, "nx, nz, ns, nt, nr, nt_interval must be >= 1 and pml0 >= 0");
-    if (!(p->dx > 0) || !(p->dz > 0) || !(p->dt > 0) || !std::isfinite(p->f0) || p->f0 == 0.f)
-        return fail(GPVI_ERR_INVALID, "dx, dz, dt must be > 0 and f0 finite and non-zero");
+    if (!(p->dx > 0) || !(p->dz > 0) || !(p->dt > 0) || !std::isfinite(p->dx) || !std::isfinite(p->dz) ||
+        !std::isfinite(p->dt) || !std::isfinite(p->f0) || p->f0 == 0.f)
+        return fail(GPVI_ERR_INVALID, "dx, dz, dt must be finite and > 0, f0 finite and non-zero");
     if (p->nr > 1 && p->dr < 1) return fail(GPVI_ERR_INVALID, "dr=%d: receivers would coincide", p->dr);
     const int lc = psm ? 0 : p->Lc + 1, pml = p->pml0 + lc;
     const long ntx = (long)p->nx + 2 * pml, ntz = (long)p->nz + 2 * pml;
