@@ -35,6 +35,7 @@ SIGNATURES = {
     "gpvi_fwi2d_batch_workspace_bytes": (_sz, [_P, _i, _i]),
     "gpvi_fwi2d_model_workspace_bytes": (_sz, [_P, _i]),
     "gpvi_fwi2d_batch": (_i, [_P, _i, _vp, _vp, _vp, _vp, _vp, _i, _i, _i, _i, _vp, _sz, _vp]),
+    "gpvi_fwi2d_batch_static": (_i, [_P, _i, _vp, _vp, _vp, _vp, _vp, _i, _i, _vp]),
     "gpvi_fwi2d_set_tuning": (_i, [_i, _i, _i]),
     "gpvi_fwi2d_get_tuning": (_i, [ctypes.POINTER(_i)] * 3),
     "gpvi_fwi2d_tile_shape": (_i, [_i, ctypes.POINTER(_i), ctypes.POINTER(_i)]),

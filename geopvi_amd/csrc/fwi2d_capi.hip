@@ -929,6 +929,67 @@ int gpvi_fwi2d_batch(const gpvi_fwi2d_params* p, int nsample, const float* vel_i
 }
 
 // ---------------------------------------------------------------------------
+// Batched solver on fixed scratch (no allocation)
+
+}  // extern "C"
+
+namespace {
+// The scratch of gpvi_fwi2d_batch_static: a fixed array in device memory,
+// one per GPU the library runs on (the code-object loader places it; it
+// stays until the process ends).  In host code the array's name is only a
+// stand-in for the symbol: its device address is obtained per device with
+// hipGetSymbolAddress and used everywhere instead.
+constexpr size_t kStaticScratchBytes = (size_t)GPVI_STATIC_SCRATCH_BYTES;
+static __device__ __align__(256) unsigned char scratch[kStaticScratchBytes];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+// Device address of the current device's scratch, looked up once per device.
+int static_scratch(void** out)
+{
+    int dev = 0;
+    HIP_TRY(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) return fail(GPVI_ERR_HIP, "device %d out of range", dev);
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* ptr = nullptr;
+        hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr)
+            return fail(GPVI_ERR_HIP, "hipGetSymbolAddress(scratch) on device %d: %s", dev, hipGetErrorString(e));
+        g_scratch_ptr[dev] = ptr;
+    }
+    *out = g_scratch_ptr[dev];
+    return 0;
+}
+}  // namespace
+
+extern "C" {
+
+int gpvi_fwi2d_batch_static(const gpvi_fwi2d_params* p, int nsample, const float* vel_inner,
+                            const float* record_obs, float* record_syn, float* grad, double* misfit, int run_fwi,
+                            int ckpt_interval, void* stream)
+{
+    int rc = validate(p);
+    if (rc) return rc;
+    if (p->laplace_slover == 1)  // the transform plans would allocate
+        return fail(GPVI_ERR_UNSUPPORTED, "gpvi_fwi2d_batch_static runs finite-difference jobs only");
+    if (nsample < 1) return fail(GPVI_ERR_INVALID, "nsample=%d", nsample);
+    const size_t need = gpvi_fwi2d_batch_workspace_bytes(p, nsample, ckpt_interval);
+    if (need > kStaticScratchBytes)
+        return fail(GPVI_ERR_WORKSPACE,
+                    "gpvi_fwi2d_batch_static: this job needs %zu bytes of scratch, the fixed scratch holds %zu "
+                    "(fewer samples or a checkpoint interval reduce it; gpvi_fwi2d_batch takes any workspace)",
+                    need, kStaticScratchBytes);
+    void* ws = nullptr;
+    if ((rc = static_scratch(&ws))) return rc;
+    // One launch per step: an instantiated graph would be an allocation.
+    return gpvi_fwi2d_batch(p, nsample, vel_inner, record_obs, record_syn, grad, misfit, run_fwi, ckpt_interval,
+                            GPVI_PHASE_ALL, GPVI_FLAG_NO_GRAPH, ws, kStaticScratchBytes, stream);
+}
+
+// ---------------------------------------------------------------------------
 // Engine
 
 }  // extern "C"

@@ -116,6 +116,23 @@ int gpvi_fwi2d_batch(const gpvi_fwi2d_params *p, int nsample, const float *vel_i
                      int run_fwi, int ckpt_interval, int phase, int flags, void *workspace,
                      size_t workspace_bytes, void *stream);
 
+/* No counterpart in the reference: for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  gpvi_fwi2d_batch with GPVI_PHASE_ALL and one launch
+ * per step, its workspace taken from a fixed device array of
+ * GPVI_STATIC_SCRATCH_BYTES per GPU (held from library load until the
+ * process ends) instead of an argument: the caller passes inputs and outputs
+ * only.  Nothing is allocated.  A job whose workspace
+ * (gpvi_fwi2d_batch_workspace_bytes) exceeds the scratch is refused with
+ * GPVI_ERR_WORKSPACE; pseudo-spectral jobs with GPVI_ERR_UNSUPPORTED.  Every
+ * call on a GPU uses that GPU's one scratch array: calls that overlap on one
+ * GPU (two streams or two threads) race and must be serialised by the
+ * caller. */
+#define GPVI_STATIC_SCRATCH_BYTES (64u << 20)
+int gpvi_fwi2d_batch_static(const gpvi_fwi2d_params *p, int nsample, const float *vel_inner,
+                            const float *record_obs, float *record_syn, float *grad, double *misfit, int run_fwi,
+                            int ckpt_interval, void *stream);
+
 /* Launch shape of the multi-unit step: tile variant (0..5), persistent grid
  * (a multiple of 8; 0 = automatic) and steps per graph segment.  Process-wide
  * defaults; the autotuner sets them. */
