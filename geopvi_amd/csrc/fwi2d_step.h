@@ -189,12 +189,24 @@ __device__ __forceinline__ FieldPtrs step_fields(const UnitDesc& u, const StepIn
 }
 
 // One time step of a macro, as seen by the points a wave updates.
+//
+// Imaging in pairs.  The reference adds each imaging step's term to the
+// gradient as it goes (fdtd2D_modelling.c:959-975), one read-modify-write of
+// the gradient per imaging step.  Here, for nt_interval k <= 2, the terms of
+// two consecutive imaging steps it + k and it of one segment (local slots
+// 2m + 1 and 2m) are applied at step it: the field of step it + k is still in
+// this step's inputs (p1 for k = 1, p0 for k = 2) and its store slot is next
+// to this one's.  The two terms are added one after the other, each rounded
+// as the reference rounds it, so the gradient is the same bits; the step at
+// the odd slot skips imaging, and the gradient is read and written once per
+// pair.  A segment with an odd number of slots images its top slot alone.
 struct StepCtx {
     int it, mode;
-    bool first_image;  // first imaging step of the adjoint pass: the gradient starts from 0
+    bool first_image;  // first gradient write of the adjoint pass: the gradient starts from 0
     bool has_save;     // wavefield store (forward-like) or imaging (adjoint) at this step
+    bool img_pair;     // imaging: also apply the term of step it + k (store slot + 1)
     bool has_ck;       // checkpoint (p0, p1) written at this step
-    Rsrc out, save, ck0, ck1;
+    Rsrc out, save, save2, ck0, ck1;
 };
 
 __device__ __forceinline__ StepCtx make_step(const UnitDesc& u, int mode, int it, int seg, float* out,
@@ -206,19 +218,25 @@ __device__ __forceinline__ StepCtx make_step(const UnitDesc& u, int mode, int it
     const bool fwd_like = mode != kAdj;
     const bool on_k = (it % u.k) == 0;
     const bool store = fwd_like && u.store && on_k && (mode == kRec || u.nseg == 1);
-    const bool image = !fwd_like && on_k;
+    const int slot = (u.nseg == 1) ? it / u.k : (it - seg * u.ckpt) / u.k;   // store slot in the segment
+    // store slots of this segment
+    const int seg_len = (u.nseg == 1) ? u.nt : (seg == u.nseg - 1 ? u.last_len : u.ckpt);
+    const int nslots = (seg_len + u.k - 1) / u.k;
+    const bool pairs = u.k <= 2;
+    const bool image = !fwd_like && on_k && !(pairs && (slot & 1));   // odd slots: applied with slot - 1
     c.has_save = store || image;
-    if (c.has_save) {
-        const int slot = (u.nseg == 1) ? it / u.k : (it - seg * u.ckpt) / u.k;
-        c.save = make_rsrc(u.save + (int64_t)slot * u.nx * u.nz);
-    }
+    c.img_pair = image && pairs && slot + 1 < nslots;
+    if (c.has_save) c.save = make_rsrc(u.save + (int64_t)slot * u.nx * u.nz);
+    if (c.img_pair) c.save2 = make_rsrc(u.save + (int64_t)(slot + 1) * u.nx * u.nz);
     c.has_ck = (mode == kFwd) && (u.nseg > 1) && (it > 0) && (it % u.ckpt == 0);
     if (c.has_ck) {
         const int j = it / u.ckpt;
         c.ck0 = make_rsrc(u.ck + (int64_t)(2 * j) * u.field_elems - margin);
         c.ck1 = make_rsrc(u.ck + (int64_t)(2 * j + 1) * u.field_elems - margin);
     }
-    c.first_image = u.grad_init && (it == ((u.nt - 1) / u.k) * u.k);
+    // the first gradient write: the highest imaging step that images, in the top segment
+    const int first_slot = pairs ? ((nslots - 1) & ~1) : nslots - 1;
+    c.first_image = u.grad_init && image && seg == u.nseg - 1 && slot == first_slot;
     c.out = make_rsrc(out - margin);
     return c;
 }
@@ -334,18 +352,22 @@ __device__ __forceinline__ void store_point(const UnitDesc& u, const PointConsts
             if (d >= 0 && d % u.dr == 0 && d / u.dr < u.nr) G(u.syn)[c.it * u.nr + d / u.dr] = val;
         }
     } else if (c.has_save && inner) {
+        // imaging: the term of step it + k first (when paired), then step it's
         const float sv = bload(c.save, gvo, go);
-        const float gprev = c.first_image ? 0.f : bload(grad, gvo, go);
-        float gnew;
+        const float sv2 = c.img_pair ? bload(c.save2, gvo, go) : 0.f;
+        const float lam2 = kc.k == 1 ? p1c : p0c;   // the field of step it + k
+        float g = c.first_image ? 0.f : bload(grad, gvo, go);
         if (kc.run_fwi) {
             const double vd = (double)v;
             const float v3 = (float)(vd * vd * vd);
-            const double term = -2.0 / (double)v3 * (double)sv * (double)val * (double)kc.k * (double)dt;
-            gnew = (float)((double)gprev + term);
+            const double coef = -2.0 / (double)v3;
+            if (c.img_pair) g = (float)((double)g + coef * (double)sv2 * (double)lam2 * (double)kc.k * (double)dt);
+            g = (float)((double)g + coef * (double)sv * (double)val * (double)kc.k * (double)dt);
         } else {
-            gnew = gprev + sv * val * (float)kc.k * dt;
+            if (c.img_pair) g = g + sv2 * lam2 * (float)kc.k * dt;
+            g = g + sv * val * (float)kc.k * dt;
         }
-        bstore(gnew, grad, gvo, go);
+        bstore(g, grad, gvo, go);
     }
     bstore(val, c.out, vo, ro);
 }
