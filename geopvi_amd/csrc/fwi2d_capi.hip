@@ -87,8 +87,10 @@ int validate(const gpvi_fwi2d_params* p)
     // The kernels address a field buffer (padded rows plus margins) and a
     // shot's records with 32-bit offsets.
     const long pitch = (ntx + 31 + kStripWidth - 1) / kStripWidth * kStripWidth;  // with the row shift
-    if (pitch * (ntz + 2 * kPadRows) * 4 >= (1L << 31))
-        return fail(GPVI_ERR_INVALID, "grid too large: a %ldx%ld padded field must stay under 2 GiB", ntz, ntx);
+    // (Offsets within a field plus one field's size, and within an inner
+    // grid plus one inner grid's size, must stay below 2^31.)
+    if (pitch * (ntz + 2 * kPadRows) * 4 >= (1L << 30) || (long)p->nx * p->nz * 4 >= (1L << 30))
+        return fail(GPVI_ERR_INVALID, "grid too large: a %ldx%ld padded field must stay under 1 GiB", ntz, ntx);
     if ((long)p->nt * p->nr >= (1L << 31))
         return fail(GPVI_ERR_INVALID, "nt*nr=%ld: a shot's records must stay under 2^31 values", (long)p->nt * p->nr);
     const long s0 = (long)pml + p->ns0, s1 = s0 + (long)(p->ns - 1) * p->ds;

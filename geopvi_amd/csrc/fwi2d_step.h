@@ -206,7 +206,8 @@ struct StepCtx {
     bool has_save;     // wavefield store (forward-like) or imaging (adjoint) at this step
     bool img_pair;     // imaging: also apply the term of step it + k (store slot + 1)
     bool has_ck;       // checkpoint (p0, p1) written at this step
-    Rsrc out, save, save2, ck0, ck1;
+    int slot4, fe4;    // byte distance to the next store slot (save + slot4: slot + 1) and to ck1 (ck0 + fe4)
+    Rsrc out, save, ck0;
 };
 
 __device__ __forceinline__ StepCtx make_step(const UnitDesc& u, int mode, int it, int seg, float* out,
@@ -227,12 +228,12 @@ __device__ __forceinline__ StepCtx make_step(const UnitDesc& u, int mode, int it
     c.has_save = store || image;
     c.img_pair = image && pairs && slot + 1 < nslots;
     if (c.has_save) c.save = make_rsrc(u.save + (int64_t)slot * u.nx * u.nz);
-    if (c.img_pair) c.save2 = make_rsrc(u.save + (int64_t)(slot + 1) * u.nx * u.nz);
+    c.slot4 = u.nx * u.nz * 4;
+    c.fe4 = (int)(u.field_elems * 4);
     c.has_ck = (mode == kFwd) && (u.nseg > 1) && (it > 0) && (it % u.ckpt == 0);
     if (c.has_ck) {
         const int j = it / u.ckpt;
-        c.ck0 = make_rsrc(u.ck + (int64_t)(2 * j) * u.field_elems - margin);
-        c.ck1 = make_rsrc(u.ck + (int64_t)(2 * j + 1) * u.field_elems - margin);
+        c.ck0 = make_rsrc(u.ck + (int64_t)(2 * j) * u.field_elems - margin);   // ck1 follows at fe4 bytes
     }
     // the first gradient write: the highest imaging step that images, in the top segment
     const int first_slot = pairs ? ((nslots - 1) & ~1) : nslots - 1;
@@ -340,7 +341,7 @@ __device__ __forceinline__ void store_point(const UnitDesc& u, const PointConsts
     if (c.mode != kAdj) {
         if (c.has_ck) {
             bstore(p0c, c.ck0, vo, ro);
-            bstore(p1c, c.ck1, vo, ro);
+            bstore(p1c, c.ck0, vo, ro + c.fe4);
         }
         if (c.has_save && inner) {
             float sv = val;
@@ -354,7 +355,7 @@ __device__ __forceinline__ void store_point(const UnitDesc& u, const PointConsts
     } else if (c.has_save && inner) {
         // imaging: the term of step it + k first (when paired), then step it's
         const float sv = bload(c.save, gvo, go);
-        const float sv2 = c.img_pair ? bload(c.save2, gvo, go) : 0.f;
+        const float sv2 = c.img_pair ? bload(c.save, gvo, go + c.slot4) : 0.f;
         const float lam2 = kc.k == 1 ? p1c : p0c;   // the field of step it + k
         float g = c.first_image ? 0.f : bload(grad, gvo, go);
         if (kc.run_fwi) {

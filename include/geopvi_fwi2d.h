@@ -70,7 +70,7 @@ int gpvi_fwi2d_read_params(const char *path, gpvi_fwi2d_params *out);
  * which has no checks): laplace_slover 0 (finite differences, Lc in 3..6) or
  * 1 (pseudo-spectral Laplacian, Lc unused and pml = pml0 as in
  * fdtd2D_modelling.c:75-85), sizes positive, every source and receiver
- * inside the padded grid, a padded field under 2 GiB and a shot's records
+ * inside the padded grid, a padded field under 1 GiB and a shot's records
  * under 2^31 values (the kernels use 32-bit offsets; GPVI_ERR_INVALID
  * otherwise).  Every entry point below runs both Laplacians
  * except the engine, which takes finite-difference jobs only; a

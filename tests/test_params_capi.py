@@ -87,10 +87,16 @@ def test_param_file_blank_lines_and_errors(tmp_path):
     (dict(dr=0), -1),
     (dict(nt=0), -1),
     (dict(dt=0.0), -1),
-    (dict(nx=24000, nz=24000), -1),        # a padded field of 2 GiB or more
+    (dict(nx=24000, nz=24000), -1),        # a padded field of 1 GiB or more
+    (dict(nx=16200, nz=16500), -1),        # just over 1 GiB
+    (dict(dx=float("inf")), -1),
+    (dict(dt=float("nan")), -1),
     (dict(nt=1 << 25, nr=64), -1),         # 2^31 record values per shot
 ])
 def test_validation_codes(change, code):
+    if change.get("nx") == 16200:
+        ok = synth.workload_params("c0", nx=15000, nz=16000).to_c()   # 15028 x 16028 padded: under 1 GiB
+        assert _lib.load().gpvi_fwi2d_validate(ctypes.byref(ok)) == 0
     p = synth.workload_params("c0", **change)
     c = p.to_c()
     assert _lib.load().gpvi_fwi2d_validate(ctypes.byref(c)) == code
