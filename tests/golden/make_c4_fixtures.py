@@ -10,7 +10,8 @@ fwi_2D (geopvi/forward/fwi2d/posterior.py:101-110), both from the oracle,
 one sample per worker process.  Recorded per iteration: the -ELBO, the
 per-sample misfits, and d(-ELBO)/dz averaged over the samples (at the
 case's seeded subset of cells, plus its norm); after the last iteration:
-the flow parameters' change (subset, norm, sum).  The observed data are
+the flow parameters' change (subset, norm, sum); the subsets are kept in
+float32.  The observed data are
 recorded by their SHA-256 only; the GPU test regenerates them.
 
     python tests/golden/make_c4_fixtures.py [--workers N]
@@ -116,9 +117,9 @@ def main():
             print(f"{method}: {time.time() - t0:.0f} s, -ELBO {loss}", flush=True)
             out[f"{method}_loss"] = np.array(loss)
             out[f"{method}_misfit"] = np.stack(record["misfit"])
-            out[f"{method}_zgrad_sub"] = np.stack(record["zgrad_sub"])
+            out[f"{method}_zgrad_sub"] = np.stack(record["zgrad_sub"]).astype(np.float32)   # small fixture
             out[f"{method}_zgrad_norm"] = np.array(record["zgrad_norm"])
-            out[f"{method}_dtheta_sub"] = d[sub_t]
+            out[f"{method}_dtheta_sub"] = d[sub_t].astype(np.float32)
             out[f"{method}_dtheta_norm"] = np.float64(np.linalg.norm(d))
             out[f"{method}_dtheta_sum"] = np.float64(d.sum())
             out[f"{method}_ntheta"] = np.int64(d.size)
