@@ -56,7 +56,8 @@ def build(verbose: bool = False) -> str:
         _run(cmd, verbose)
         objs.append(obj)
     tmp = LIB + ".tmp"
-    _run([_hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs, "-lhipfft"], verbose)
+    _run([_hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs, "-lhipfft",
+          "-lrocprofiler-sdk-roctx"], verbose)
     os.replace(tmp, LIB)
     return LIB
 

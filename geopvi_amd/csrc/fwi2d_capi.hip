@@ -3,6 +3,7 @@
 
 #include <hip/hip_runtime.h>
 #include <hipfft/hipfft.h>
+#include <rocprofiler-sdk-roctx/roctx.h>
 
 #include <algorithm>
 #include <climits>
@@ -35,6 +36,24 @@ int fail(int code, const char* fmt, ...)
     g_err = buf;
     return code;
 }
+
+// roctx ranges (host side, visible with rocprofv3 --marker-trace): one per
+// batched call, graph segment launch and engine advance; a mark per finished
+// engine job.  Without a profiler attached they cost a function call.
+struct TraceRange {
+    explicit TraceRange(const char* fmt, ...)
+    {
+        char buf[160];
+        va_list ap;
+        va_start(ap, fmt);
+        vsnprintf(buf, sizeof buf, fmt, ap);
+        va_end(ap);
+        roctxRangePushA(buf);
+    }
+    ~TraceRange() { roctxRangePop(); }
+    TraceRange(const TraceRange&) = delete;
+    TraceRange& operator=(const TraceRange&) = delete;
+};
 
 #define HIP_TRY(expr)                                                                        \
     do {                                                                                     \
@@ -522,6 +541,7 @@ int get_graph_locked(const GraphKey& key, hipGraphExec_t* out)
 // executable graph is never launched after another thread destroyed it.
 int run_graph(const GraphKey& key, hipStream_t st, int reps)
 {
+    TraceRange tr("fwi2d step segments: %d x %d steps", reps, key.steps);
     std::lock_guard<std::mutex> lk(g_graph_mu);
     hipGraphExec_t exec;
     int rc = get_graph_locked(key, &exec);
@@ -872,6 +892,8 @@ int gpvi_fwi2d_batch(const gpvi_fwi2d_params* p, int nsample, const float* vel_i
     if (rc) return rc;
     if (nsample < 1) return fail(GPVI_ERR_INVALID, "nsample=%d", nsample);
     if (phase < GPVI_PHASE_ALL || phase > GPVI_PHASE_ADJOINT) return fail(GPVI_ERR_INVALID, "phase=%d", phase);
+    TraceRange tr("gpvi_fwi2d_batch %dx%d x %d samples x %d shots, %d steps, phase %d, ckpt %d", p->nz, p->nx,
+                  nsample, p->ns, p->nt, phase, ckpt_interval);
     if (!record_obs || !record_syn || (phase != GPVI_PHASE_ADJOINT && !vel_inner) ||
         (phase != GPVI_PHASE_FORWARD && !grad))
         return fail(GPVI_ERR_INVALID, "NULL buffer");
@@ -1218,6 +1240,7 @@ int gpvi_engine_submit(gpvi_engine* e, const gpvi_fwi2d_params* p, int nsample, 
 int gpvi_engine_advance(gpvi_engine* e, int nsteps)
 {
     if (!e) return fail(GPVI_ERR_INVALID, "NULL engine");
+    TraceRange tr("gpvi_engine_advance %zu jobs", e->active.size());
     int rc;
     if (e->dirty && (rc = engine_rebuild(e))) return rc;
     if (e->active.empty() || nsteps <= 0) return 0;
@@ -1249,6 +1272,11 @@ int gpvi_engine_advance(gpvi_engine* e, int nsteps)
         EngineJob& jb = e->active[i];
         if (jb.end_step <= e->gstep) {
             if ((rc = engine_finalize(e, jb))) return rc;
+            {
+                char msg[64];
+                std::snprintf(msg, sizeof msg, "engine job %lld finalised", jb.id);
+                roctxMarkA(msg);
+            }
             engine_free_slots(e, jb.j.unit0, jb.nunit);
             e->finished[jb.id] = jb.done;
             e->active.erase(e->active.begin() + i);
