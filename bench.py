@@ -105,7 +105,7 @@ def stream_3r1w_gbs(dev, mib: int = 512, reps: int = 10) -> float:
     return 4 * n * 4 / sec / 1e9
 
 
-def serve_stream(args, rate, nreq, seed, dev, mix="bench"):
+def serve_stream(args, rate, nreq, seed, dev, mix="bench", compare_batches=False, **server_kw):
     """A Poisson stream of misfit+gradient requests through the server.
 
     Requests follow one of the serving generator's mixes: "bench" (grids
@@ -132,7 +132,7 @@ def serve_stream(args, rate, nreq, seed, dev, mix="bench"):
     torch.cuda.synchronize()
     total_bytes = sum(m.shape[0] * p.ns * algorithmic_bytes_per_unit(p) for p, m, _ in reqs)
     total_samples = sum(m.shape[0] for _, m, _ in reqs)
-    with FWIServer(dev.index or 0) as srv:
+    with FWIServer(dev.index or 0, **server_kw) as srv:
         t0 = time.perf_counter()
         t_next = t0
         futs = []
@@ -154,7 +154,23 @@ def serve_stream(args, rate, nreq, seed, dev, mix="bench"):
     span = t_end - t0
     n_full = sum(1 for r in res if r.extra.get("ckpt_interval") == 0)
     n_split = sum(1 for r in res if r.extra.get("chunks", 1) > 1)
-    return {"requests": nreq, "rate_req_s": rate, "store": f"auto: {n_full} full store, {nreq - n_full} checkpointed",
+    batches = None
+    if compare_batches:
+        # The same requests as back-to-back batched calls (full store), for
+        # the batch path's rate on exactly this work.
+        del res, futs
+        for p, vel, obs in reqs:
+            solver.fwi_batch(p, vel[:1], obs)
+        torch.cuda.synchronize()
+        t_b = time.perf_counter()
+        for p, vel, obs in reqs:
+            solver.fwi_batch(p, vel, obs)
+        torch.cuda.synchronize()
+        tb = time.perf_counter() - t_b
+        batches = {"achieved_GBs": round(total_bytes / tb / 1e9, 1), "seconds": round(tb, 3),
+                   "samples_per_s": round(total_samples / tb, 3)}
+    policy = server_kw.get("store_policy", "auto")
+    return {"requests": nreq, "rate_req_s": rate, "store": f"{policy}: {n_full} full store, {nreq - n_full} checkpointed",
             "split_requests": n_split,
             "arena_waits": arena_waits,
             "p50_ms": round(float(np.percentile(lat, 50)) * 1e3, 1),
@@ -163,7 +179,8 @@ def serve_stream(args, rate, nreq, seed, dev, mix="bench"):
             "admission_wait_p50_ms": round(float(np.percentile(wait, 50)) * 1e3, 1),
             "makespan_s": round(span, 3),
             "samples_per_s": round(total_samples / span, 3),
-            "achieved_GBs": round(total_bytes / span / 1e9, 1)}
+            "achieved_GBs": round(total_bytes / span / 1e9, 1),
+            **({"same_jobs_as_batches": batches} if batches else {})}
 
 
 def vi_leg(args, dev):
@@ -508,7 +525,8 @@ def main():
 
     if not args.no_serve:
         serve = {"mix": "bench (grids 100x200/200x400, 4-16 samples, 5-10 shots, 500-1000 steps)"}
-        serve["saturated"] = serve_stream(args, args.serve_rate, args.serve_requests, rng_seed + 300, dev)
+        serve["saturated"] = serve_stream(args, args.serve_rate, args.serve_requests, rng_seed + 300, dev,
+                                          compare_batches=True)
         log(f"served stream at {args.serve_rate}/s done")
         if args.serve_low_rate > 0:
             serve["sub_saturation"] = serve_stream(args, args.serve_low_rate, args.serve_low_requests,

@@ -202,3 +202,19 @@ def test_run_fwi_0_gradient_bilinear_and_zero(oracle_mod):
     assert not g0.any()
     # the records of run_fwi=0 are the forward records
     assert np.array_equal(syn0, oracle_mod.forward(p, vs, dtype=np.float64))
+
+
+def test_row_major_build_gives_the_same_bits(oracle_mod):
+    """The oracle's row-major build (used for long fixture runs) computes the
+    same arithmetic in another loop order: identical records and gradients."""
+    import oracle
+    p, vt, vs = interval_case()
+    obs = oracle.forward(p, vt)
+    prev = oracle.select(row_major=True)
+    try:
+        rm = oracle.fwi(p, vs, obs), oracle.fwi(p, vs, obs, run_fwi=0)
+    finally:
+        oracle.use_library(prev)
+    ref = oracle.fwi(p, vs, obs), oracle.fwi(p, vs, obs, run_fwi=0)
+    for (s1, g1), (s2, g2) in zip(rm, ref):
+        assert np.array_equal(s1, s2) and np.array_equal(g1, g2)
