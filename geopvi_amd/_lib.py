@@ -17,6 +17,7 @@ GPVI_PHASE_FORWARD = 1
 GPVI_PHASE_ADJOINT = 2
 GPVI_FLAG_NO_GRAPH = 1
 GPVI_FLAG_NO_STORE = 2
+GPVI_FLAG_ONE_STREAM = 4
 GPVI_ERR_CAPACITY = -7  # engine unit/tile tables full (include/geopvi_fwi2d.h)
 
 # Every exported symbol and its ctypes signature: (restype, argtypes).
@@ -41,6 +42,8 @@ SIGNATURES = {
     "gpvi_fwi2d_tile_shape": (_i, [_i, ctypes.POINTER(_i), ctypes.POINTER(_i)]),
     "gpvi_fwi2d_set_time_blocking": (_i, [_i]),
     "gpvi_fwi2d_get_time_blocking": (_i, []),
+    "gpvi_fwi2d_set_lanes": (_i, [_i]),
+    "gpvi_fwi2d_get_lanes": (_i, []),
     "gpvi_engine_create": (_i, [_vp, _i, _i, ctypes.POINTER(_vp)]),
     "gpvi_engine_destroy": (_i, [_vp]),
     "gpvi_engine_submit": (_i, [_vp, _P, _i, _vp, _vp, _vp, _vp, _vp, _i, _i, _vp, _sz,
@@ -115,6 +118,15 @@ def set_time_blocking(steps_per_launch: int) -> None:
 
 def get_time_blocking() -> int:
     return int(load().gpvi_fwi2d_get_time_blocking())
+
+
+def set_lanes(lanes: int) -> None:
+    """1 or 2: batched calls advance two halves of their units on two streams."""
+    check(load().gpvi_fwi2d_set_lanes(int(lanes)))
+
+
+def get_lanes() -> int:
+    return int(load().gpvi_fwi2d_get_lanes())
 
 
 def tile_shape(variant: int):

@@ -61,11 +61,14 @@ struct TraceRange {
         if (e_ != hipSuccess) return fail(GPVI_ERR_HIP, "%s: %s", #expr, hipGetErrorString(e_)); \
     } while (0)
 
+constexpr int kMaxLanes = 4;
+
 struct Tuning {
     int variant = 4;       // 24-row strip segments: fastest measured on the 20-shot 300x100 batch
     int grid = 0;          // 0: automatic
     int graph_steps = 32;  // steps per graph segment
     int tb = 1;            // time steps per multi-unit launch (1 or 2)
+    int lanes = 2;         // batched calls: unit groups advanced on as many streams (1..kMaxLanes)
 };
 Tuning g_tuning;
 std::mutex g_tuning_mu;
@@ -364,7 +367,7 @@ void batch_layout(const Geom& g, const Sched& sc, int nsample, bool tables, char
     if (tables) {
         L->units = b.take<UnitDesc>((size_t)U);
         L->tiles = b.take<TileRef>((size_t)(U * g.tpu));
-        L->ctl = b.take<StepCtl>(1);
+        L->ctl = b.take<StepCtl>(kMaxLanes);   // one per lane
     } else {
         L->units = nullptr; L->tiles = nullptr; L->ctl = nullptr;
     }
@@ -490,6 +493,17 @@ std::map<GraphKey, hipGraphExec_t> g_graphs;
 std::map<int, hipStream_t> g_capture_streams;
 constexpr size_t kMaxGraphs = 64;
 
+// Empty the cache (after a device-wide synchronisation) when `fresh` new
+// segments would not fit; called with g_graph_mu held.
+int make_room_locked(size_t fresh)
+{
+    if (g_graphs.size() + fresh <= kMaxGraphs) return 0;
+    HIP_TRY(hipDeviceSynchronize());
+    for (auto& kv : g_graphs) (void)hipGraphExecDestroy(kv.second);
+    g_graphs.clear();
+    return 0;
+}
+
 // Find or build the segment for `key`; called with g_graph_mu held.
 int get_graph_locked(const GraphKey& key, hipGraphExec_t* out)
 {
@@ -498,11 +512,8 @@ int get_graph_locked(const GraphKey& key, hipGraphExec_t* out)
         *out = it->second;
         return 0;
     }
-    if (g_graphs.size() >= kMaxGraphs) {
-        HIP_TRY(hipDeviceSynchronize());
-        for (auto& kv : g_graphs) (void)hipGraphExecDestroy(kv.second);
-        g_graphs.clear();
-    }
+    int rc = make_room_locked(1);
+    if (rc) return rc;
     hipStream_t cap;
     auto sit = g_capture_streams.find(key.device);
     if (sit == g_capture_streams.end()) {
@@ -547,6 +558,66 @@ int run_graph(const GraphKey& key, hipStream_t st, int reps)
     int rc = get_graph_locked(key, &exec);
     if (rc) return rc;
     for (int r = 0; r < reps; ++r) HIP_TRY(hipGraphLaunch(exec, st));
+    return 0;
+}
+
+// The segments of `n` lanes (one graph and one stream each), enqueued in
+// turn so every lane's stream has work queued from the start.
+int run_graph_lanes(const GraphKey* keys, const hipStream_t* st, int n, int reps)
+{
+    TraceRange tr("fwi2d step segments: %d lanes x %d x %d steps", n, reps, keys[0].steps);
+    std::lock_guard<std::mutex> lk(g_graph_mu);
+    // Room for every lane's segment first: emptying the cache between two
+    // lookups would destroy a segment already looked up.
+    size_t fresh = 0;
+    for (int l = 0; l < n; ++l) fresh += g_graphs.count(keys[l]) ? 0 : 1;
+    int rc = make_room_locked(fresh);
+    if (rc) return rc;
+    hipGraphExec_t exec[kMaxLanes];
+    for (int l = 0; l < n; ++l)
+        if ((rc = get_graph_locked(keys[l], &exec[l]))) return rc;
+    for (int r = 0; r < reps; ++r)
+        for (int l = 0; l < n; ++l) HIP_TRY(hipGraphLaunch(exec[l], st[l]));
+    return 0;
+}
+
+// Companion streams of the batched solver's lanes 1.. : one per (device,
+// caller stream, lane), created on first use with the default priority and
+// kept until the process ends.  Past kMaxCompanions of them, new callers run
+// one lane.
+constexpr size_t kMaxCompanions = 96;
+std::mutex g_companion_mu;
+std::map<std::tuple<int, hipStream_t, int>, hipStream_t> g_companions;
+
+int companion_stream(int device, hipStream_t st, int lane, hipStream_t* out)
+{
+    std::lock_guard<std::mutex> lk(g_companion_mu);
+    auto key = std::make_tuple(device, st, lane);
+    auto it = g_companions.find(key);
+    if (it != g_companions.end()) {
+        *out = it->second;
+        return 0;
+    }
+    if (g_companions.size() >= kMaxCompanions) {
+        *out = nullptr;
+        return 0;
+    }
+    hipStream_t s2;
+    HIP_TRY(hipStreamCreateWithFlags(&s2, hipStreamNonBlocking));
+    g_companions[key] = s2;
+    *out = s2;
+    return 0;
+}
+
+// `to` waits for the work enqueued on `from` so far.
+int stream_wait(hipStream_t to, hipStream_t from)
+{
+    hipEvent_t ev;
+    HIP_TRY(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+    hipError_t e = hipEventRecord(ev, from);
+    if (e == hipSuccess) e = hipStreamWaitEvent(to, ev, 0);
+    (void)hipEventDestroy(ev);   // released once the recorded work completes
+    if (e != hipSuccess) return fail(GPVI_ERR_HIP, "stream fork/join: %s", hipGetErrorString(e));
     return 0;
 }
 
@@ -615,6 +686,16 @@ int gpvi_fwi2d_set_time_blocking(int steps_per_launch)
 }
 
 int gpvi_fwi2d_get_time_blocking(void) { return tuning().tb; }
+
+int gpvi_fwi2d_set_lanes(int lanes)
+{
+    if (lanes < 1 || lanes > kMaxLanes) return fail(GPVI_ERR_INVALID, "lanes=%d: 1 to %d", lanes, kMaxLanes);
+    std::lock_guard<std::mutex> lk(g_tuning_mu);
+    g_tuning.lanes = lanes;
+    return 0;
+}
+
+int gpvi_fwi2d_get_lanes(void) { return tuning().lanes; }
 
 int gpvi_fwi2d_get_tuning(int* variant, int* grid, int* graph_steps)
 {
@@ -922,6 +1003,37 @@ int gpvi_fwi2d_batch(const gpvi_fwi2d_params* p, int nsample, const float* vel_i
     if (phase == GPVI_PHASE_FORWARD && !model_only) layout_stamp_put(workspace, p, nsample, ckpt_interval, tn);
     const int store = (phase == GPVI_PHASE_FORWARD && (flags & GPVI_FLAG_NO_STORE)) ? 0 : 1;
     JobDesc j = make_job(p, g, sc, nsample, L, record_obs, record_syn, run_fwi, phase, store);
+    // Lanes: the units split into groups with contiguous runs of the tile
+    // list (groups of samples, or the two shot halves of a single sample),
+    // each advanced by its own launches on its own stream.
+    const int per_unit = g.tpu;
+    int nlane = 1;
+    int lane_t0[kMaxLanes] = {0}, lane_n[kMaxLanes] = {nsample * g.ns * per_unit};
+    hipStream_t lane_st[kMaxLanes] = {st};
+    if (g.lc > 0 && tn.lanes > 1 && !(flags & GPVI_FLAG_ONE_STREAM) && nsample * g.ns >= 2) {
+        const int want = nsample >= 2 ? std::min(tn.lanes, nsample) : 2;
+        int have = 1;
+        while (have < want) {
+            if ((rc = companion_stream(current_device(), st, have, &lane_st[have]))) return rc;
+            if (!lane_st[have]) break;
+            ++have;
+        }
+        if (have > 1) {
+            nlane = have;
+            if (nsample == 1) {
+                j.shot_split = g.ns / 2;
+                lane_n[0] = j.shot_split * per_unit;
+                lane_t0[1] = lane_n[0];
+                lane_n[1] = (g.ns - j.shot_split) * per_unit;
+            } else {
+                for (int l = 0; l < nlane; ++l) {
+                    const int m0 = l * nsample / nlane, m1 = (l + 1) * nsample / nlane;
+                    lane_t0[l] = m0 * g.ns * per_unit;
+                    lane_n[l] = (m1 - m0) * g.ns * per_unit;
+                }
+            }
+        }
+    }
     HIP_TRY(launch_build_units(st, j, L.units, L.tiles));
     if (g.lc == 0) {
         if ((rc = psm_run(st, g, sc, j, L, nsample * g.ns))) return rc;
@@ -932,19 +1044,40 @@ int gpvi_fwi2d_batch(const gpvi_fwi2d_params* p, int nsample, const float* vel_i
                                   misfit));
         return 0;
     }
-    const int ntiles = nsample * g.ns * g.tpu;
-    int n4[4] = {0, 0, 0, 0}, o4[4] = {0, 0, 0, 0};
-    n4[g.lc - 4] = ntiles;
-    HIP_TRY(launch_ctl_set(st, L.ctl, 0, n4, o4));
-    const int grid = launch_grid(tn.grid, ntiles, g.lc, g.tb);
+    int grid[kMaxLanes];
+    for (int l = 0; l < nlane; ++l) {
+        int n4[4] = {0, 0, 0, 0}, o4[4] = {0, 0, 0, 0};
+        n4[g.lc - 4] = lane_n[l];
+        o4[g.lc - 4] = lane_t0[l];
+        HIP_TRY(launch_ctl_set(st, L.ctl + l, 0, n4, o4));
+        grid[l] = launch_grid(tn.grid, lane_n[l], g.lc, g.tb);
+    }
+    // Fork: the other lanes start after everything enqueued on the caller's
+    // stream so far (job set-up, unit table, step counters).
+    for (int l = 1; l < nlane; ++l)
+        if ((rc = stream_wait(lane_st[l], st))) return rc;
     if (flags & GPVI_FLAG_NO_GRAPH) {
-        for (int s = 0; s < j.total; ++s)
-            HIP_TRY(launch_step_multi(g.lc, g.tb, grid, st, L.units, L.tiles, L.ctl, s));
+        for (int s = 0; s < j.total && !rc; ++s)
+            for (int l = 0; l < nlane && !rc; ++l) {
+                const hipError_t e = launch_step_multi(g.lc, g.tb, grid[l], lane_st[l], L.units, L.tiles, L.ctl + l, s);
+                if (e != hipSuccess) rc = fail(GPVI_ERR_HIP, "step launch: %s", hipGetErrorString(e));
+            }
     } else {
         const int seg = std::min(tn.graph_steps, j.total);
-        GraphKey key{L.units, L.tiles, L.ctl, grid, g.tb, 1 << (g.lc - 4), seg, current_device(), {0, 0, 0, 0}};
-        if ((rc = run_graph(key, st, (j.total + seg - 1) / seg))) return rc;
+        GraphKey keys[kMaxLanes];
+        for (int l = 0; l < nlane; ++l)
+            keys[l] = GraphKey{L.units, L.tiles, L.ctl + l, grid[l], g.tb, 1 << (g.lc - 4), seg, current_device(),
+                               {0, 0, 0, 0}};
+        rc = nlane == 1 ? run_graph(keys[0], st, (j.total + seg - 1) / seg)
+                        : run_graph_lanes(keys, lane_st, nlane, (j.total + seg - 1) / seg);
     }
+    // Join: the caller's stream continues after every lane (also when a
+    // launch failed, so no lane work is left behind the caller's stream).
+    for (int l = 1; l < nlane; ++l) {
+        const int rj = stream_wait(st, lane_st[l]);
+        if (!rc) rc = rj;
+    }
+    if (rc) return rc;
     if (phase != GPVI_PHASE_FORWARD)
         HIP_TRY(launch_grad_reduce(st, L.gpriv, L.grad_stride, grad, g.ns, g.ni, nsample));
     if (phase != GPVI_PHASE_ADJOINT && misfit)
@@ -1008,9 +1141,11 @@ int gpvi_fwi2d_batch_static(const gpvi_fwi2d_params* p, int nsample, const float
                     need, kStaticScratchBytes);
     void* ws = nullptr;
     if ((rc = static_scratch(&ws))) return rc;
-    // One launch per step: an instantiated graph would be an allocation.
+    // One launch per step on the caller's stream: an instantiated graph, a
+    // companion stream or its events would be allocations.
     return gpvi_fwi2d_batch(p, nsample, vel_inner, record_obs, record_syn, grad, misfit, run_fwi, ckpt_interval,
-                            GPVI_PHASE_ALL, GPVI_FLAG_NO_GRAPH, ws, kStaticScratchBytes, stream);
+                            GPVI_PHASE_ALL, GPVI_FLAG_NO_GRAPH | GPVI_FLAG_ONE_STREAM, ws, kStaticScratchBytes,
+                            stream);
 }
 
 // ---------------------------------------------------------------------------

@@ -26,6 +26,7 @@ struct JobDesc {
     float dt, dx, dz, rdt;
     float rdx, rdz;
     int fastdiv, band_tz;
+    int shot_split;       // > 0: a sample's shots [0, shot_split) and [shot_split, ns) list their tiles apart
     float fd[8];
     int64_t fe;
     const float* wtab;

@@ -129,16 +129,31 @@ __global__ void build_units_kernel(JobDesc j, UnitDesc* __restrict__ units, Tile
     const int nunit = j.nsample * j.ns;
     const int tpu = 2 * j.band_tz + j.tiles_x * j.tiles_z;
     const int64_t ntile = (int64_t)nunit * tpu;
-    // Item order: sample, item position, shot.  The shots of one sample read
-    // the same velocity tile at the same item position; next to each other
-    // in the list, they run at about the same time on the same XCD, so the
-    // tile comes from HBM once and from L2 for the other shots.
+    // Item order: sample, shot group, item position, shot.  The shots of one
+    // sample read the same velocity tile at the same item position; next to
+    // each other in the list, they run at about the same time on the same
+    // XCD, so the tile comes from HBM once and from L2 for the other shots.
+    // With shot_split > 0 the shots [0, shot_split) and [shot_split, ns) of a
+    // sample form two groups, each with a contiguous run of the list (a batch
+    // of one sample advances them as two lanes on two streams).
+    const int64_t per_sample = (int64_t)j.ns * tpu;
+    const int64_t split_tiles = (int64_t)j.shot_split * tpu;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < ntile;
          i += (int64_t)gridDim.x * blockDim.x) {
-        const int64_t mt = i / j.ns;                       // sample * tpu + item position
-        const int sh = (int)(i - mt * j.ns);
-        const int m = (int)(mt / tpu);
-        const int t = (int)(mt - (int64_t)m * tpu);
+        const int m = (int)(i / per_sample);
+        int64_t r = i - (int64_t)m * per_sample;   // position in the sample's run
+        int g0 = 0, gn = j.ns;                     // the shot group: shots [g0, g0 + gn)
+        if (j.shot_split > 0) {
+            if (r < split_tiles) {
+                gn = j.shot_split;
+            } else {
+                r -= split_tiles;
+                g0 = j.shot_split;
+                gn = j.ns - j.shot_split;
+            }
+        }
+        const int t = (int)(r / gn);
+        const int sh = g0 + (int)(r - (int64_t)t * gn);
         tiles[j.tile0 + i] = TileRef{j.unit0 + m * j.ns + sh, t};
     }
     const int un = blockIdx.x * blockDim.x + threadIdx.x;

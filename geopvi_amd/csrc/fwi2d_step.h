@@ -639,10 +639,16 @@ __device__ __forceinline__ void process_item(const UnitDesc& u, int s, int item,
             }
             __builtin_amdgcn_wave_barrier();
             if (owned) {
+                if (!cA.has_save && !cA.has_ck) {
+                    // a plain step (no store, checkpoint or imaging): the field only
 #pragma unroll
-                for (int r = 0; r < R; ++r)
-                    store_point(u, kc, cA, val[r], p0r[r], w1[r + H], vh[H + r], ix, vo, row_off(za0 + r), gvo,
-                                (za0 + r - pml) * nx4, rg, false, inner_x);
+                    for (int r = 0; r < R; ++r) bstore(val[r], cA.out, vo, row_off(za0 + r));
+                } else {
+#pragma unroll
+                    for (int r = 0; r < R; ++r)
+                        store_point(u, kc, cA, val[r], p0r[r], w1[r + H], vh[H + r], ix, vo, row_off(za0 + r), gvo,
+                                    (za0 + r - pml) * nx4, rg, false, inner_x);
+                }
             }
         } else
 #pragma unroll

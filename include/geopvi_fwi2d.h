@@ -46,6 +46,8 @@ extern "C" {
 /* Flags of gpvi_fwi2d_batch. */
 #define GPVI_FLAG_NO_GRAPH 1  /* one launch per step instead of replayed graph segments */
 #define GPVI_FLAG_NO_STORE 2  /* forward modelling only: no wavefield store (PHASE_FORWARD) */
+#define GPVI_FLAG_ONE_STREAM 4 /* all launches on the caller's stream (no second lane; see
+                                  gpvi_fwi2d_set_lanes) */
 
 /*
  * Run parameters: the 19 values of the reference's positional parameter
@@ -144,6 +146,17 @@ int gpvi_fwi2d_tile_shape(int variant, int *tile_x, int *tile_z);
  * second computed from the first in registers/LDS (temporal blocking). */
 int gpvi_fwi2d_set_time_blocking(int steps_per_launch);
 int gpvi_fwi2d_get_time_blocking(void);
+/* Lanes of a batched call (1 or 2; default 2).  With 2, the (sample, shot)
+ * units are split into two halves (by samples, or by shots for one sample),
+ * each advanced by its own step launches on its own stream: the caller's
+ * stream and a companion stream the library creates once per caller stream
+ * (default priority), joined back to the caller's stream by events.  The
+ * step launches of one half then fill the GPU while those of the other
+ * drain.  Results are the same bits for 1 and 2.  Process-wide; 1 is
+ * always used with GPVI_FLAG_ONE_STREAM, for the pseudo-spectral path, for
+ * a single unit and by gpvi_fwi2d_batch_static. */
+int gpvi_fwi2d_set_lanes(int lanes);
+int gpvi_fwi2d_get_lanes(void);
 
 /* ---- Engine: continuous batching of misfit+gradient jobs ----
  * Jobs of any grid size join the running set at a step boundary; every step

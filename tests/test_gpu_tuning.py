@@ -40,11 +40,12 @@ def small(oracle_mod):
 
 @pytest.fixture()
 def restore_tuning():
-    saved = (_lib.get_tuning(), _lib.get_time_blocking())
+    saved = (_lib.get_tuning(), _lib.get_time_blocking(), _lib.get_lanes())
     yield
-    (v, g, s), tb = saved
+    (v, g, s), tb, lanes = saved
     _lib.set_tuning(v, g, s)
     _lib.set_time_blocking(tb)
+    _lib.set_lanes(lanes)
 
 
 @pytest.mark.parametrize("ckpt", [0, 60])
@@ -82,3 +83,52 @@ def test_autotune_small(small, restore_tuning):
     # applied process-wide
     v, g, _ = _lib.get_tuning()
     assert (v, g, _lib.get_time_blocking()) == (res["variant"], res["grid"], res["tb"])
+
+
+@pytest.mark.parametrize("ckpt", [0, 60])
+def test_lanes_bit_identical(small, restore_tuning, ckpt):
+    """Units advanced on 1 to 4 streams (groups of samples, or the two shot
+    halves of one sample), with graph segments or one launch per step, give
+    the bits of a single lane."""
+    p, vel, obs, _, _ = small
+    assert p.ns == 3   # odd: the shot halves of one sample are uneven
+    vel3 = np.concatenate([vel, vel[:1] * 1.01])
+    obs = solver.forward_batch(p, dev(vel[1:2]))[0].contiguous()
+    for nsample in (1, 2, 3):
+        vel_t = dev(vel3[:nsample])
+        ref = None
+        for lanes in (1, 2, 3, 4):
+            for graph in (True, False):
+                _lib.set_lanes(lanes)
+                out = [host(t) for t in solver.fwi_batch(p, vel_t, obs, ckpt_interval=ckpt, graph=graph)]
+                tag = f"nsample={nsample} lanes={lanes} graph={graph} ckpt={ckpt}"
+                print(tag, flush=True)
+                if ref is None:
+                    ref = out
+                else:
+                    for a, b in zip(out, ref):
+                        assert np.array_equal(a, b), tag
+
+
+def test_lanes_phase_split_and_one_stream(small, restore_tuning):
+    """The autograd path (forward and adjoint phases in separate calls) and
+    GPVI_FLAG_ONE_STREAM give the single-lane bits."""
+    p, vel, obs, _, _ = small
+    vel_t, obs_t = dev(vel), dev(obs)
+    _lib.set_lanes(1)
+    syn1, grad1, mis1 = (host(t) for t in solver.fwi_batch(p, vel_t, obs_t, ckpt_interval=60))
+    _lib.set_lanes(2)
+    v = vel_t.clone().requires_grad_()
+    loss = solver.fwi2d_misfit(v, obs_t, p, ckpt_interval=60)
+    loss.sum().backward()
+    assert np.array_equal(host(loss), mis1.astype(host(loss).dtype))
+    assert np.array_equal(host(v.grad), -grad1)
+    n = vel.shape[0]
+    syn = torch.empty(n, p.record_len, device="cuda")
+    grad = torch.zeros(n, p.ni, device="cuda")
+    mis = torch.empty(n, dtype=torch.float64, device="cuda")
+    ws = torch.empty(solver.batch_workspace_bytes(p, n, 60), dtype=torch.uint8, device="cuda")
+    solver._batch_call(p, n, vel_t, obs_t, syn, grad, mis, 1, 60, _lib.GPVI_PHASE_ALL, _lib.GPVI_FLAG_ONE_STREAM,
+                       ws, vel_t.device)
+    assert np.array_equal(host(syn), syn1) and np.array_equal(host(grad), grad1)
+    assert np.array_equal(host(mis), mis1)

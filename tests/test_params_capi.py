@@ -142,6 +142,21 @@ def test_tuning_round_trip():
         _lib.set_tuning(v, g, s)
 
 
+def test_lanes_round_trip():
+    lanes = _lib.get_lanes()
+    assert lanes == 2   # the default: two streams per batched call
+    try:
+        for n in (1, 2, 3, 4):
+            _lib.set_lanes(n)
+            assert _lib.get_lanes() == n
+        for bad in (0, 5, -1):
+            with pytest.raises(_lib.GpviError):
+                _lib.set_lanes(bad)
+        assert _lib.get_lanes() == 4
+    finally:
+        _lib.set_lanes(lanes)
+
+
 def test_compat_entry_reports_bad_parameter_file(tmp_path):
     lib = _lib.load()
     buf = (ctypes.c_float * 4)()
