@@ -52,6 +52,7 @@ def parse():
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--tune", default="", help="variant,grid,steps_per_launch: apply instead of autotuning")
+    ap.add_argument("--lanes", type=int, default=0, help="streams per batched call (1..4; 0 = the library's default)")
     ap.add_argument("--copy-calibration-mb", type=int, default=0,
                     help="also run a plain device copy of this many MiB (HBM counter calibration)")
     ap.add_argument("--serve-requests", type=int, default=200)
@@ -380,6 +381,8 @@ def main():
     from geopvi_amd.forward.fwi2d import solver
 
     _lib.load()
+    if args.lanes:
+        _lib.set_lanes(args.lanes)
     dev = torch.device("cuda", 0)
     torch.cuda.set_device(dev)
     p = synth.workload_params("c1")
@@ -421,7 +424,10 @@ def main():
                 tune = {**tune, "applied": key, "note": "recorded-traffic shape within 3 % of the fastest"}
     variant, grid, graph_steps = _lib.get_tuning()
     tb = _lib.get_time_blocking()
-    log(f"launch shape: variant {variant} grid {grid} steps/launch {tb}")
+    # Streams the batch's units are split over (gpvi_fwi2d_set_lanes): every
+    # macro step is then one concurrent dispatch per lane.
+    lanes = min(_lib.get_lanes(), args.nsample) if args.nsample >= 2 else min(_lib.get_lanes(), 2)
+    log(f"launch shape: variant {variant} grid {grid} steps/launch {tb} lanes {lanes}")
 
     wsb = solver.batch_workspace_bytes(p, args.nsample, args.ckpt)
     ws = torch.empty(wsb, dtype=torch.uint8, device=dev)
@@ -449,12 +455,15 @@ def main():
     ms_per_step = wall * 1e3 / args.steps
     value = args.nsample * args.steps / wall
 
-    # Roofline of the dominant kernel (multi-unit step): algorithmic bytes
-    # of one launch / average launch time.  The step launches are all but a
-    # few small set-up and reduction kernels of the timed region, so the
-    # region's GPU time over the launch count bounds the step kernel's
-    # average duration from above (the profile under profiles/ gives it
-    # exactly).
+    # Roofline of the dominant kernel (multi-unit step).  A launch here is
+    # one macro step of every unit: `lanes` concurrent dispatches, each over
+    # its share of the units, on the lanes' streams.  Algorithmic bytes of
+    # one launch / its average time, the timed region's time (events on the
+    # caller's stream, which joins the lanes' streams at the end of every
+    # evaluation) over the launch count.  The step launches are all but a
+    # few small set-up and reduction kernels of the region, so this bounds
+    # the step's average time from above; the profile under profiles/ gives
+    # the dispatches' union per launch from the kernel trace.
     sch = solver.schedule(p, args.ckpt, tb)
     units = args.nsample * p.ns
     launches = sch["launches"]
@@ -463,7 +472,8 @@ def main():
     achieved = bytes_per_launch / avg_launch_s / 1e9
     traffic = None
     tj = _recorded_traffic()
-    if tj and tj.get("tb") == tb and tj.get("variant") == variant and tj.get("ckpt") == args.ckpt:
+    if (tj and tj.get("tb") == tb and tj.get("variant") == variant and tj.get("ckpt") == args.ckpt
+            and tj.get("lanes", 1) == lanes):
         traffic = tj.get("hbm_bytes_per_launch")
 
     out = {
@@ -483,11 +493,12 @@ def main():
                                "2000 steps, dt=1ms, nt_interval=2, fp32, batched forward+adjoint",
                    "nsample": args.nsample, "shots": p.ns, "nt": p.nt, "grid": [p.nz, p.nx],
                    "ckpt_interval": args.ckpt, "tile_variant": variant, "launch_grid": grid,
-                   "steps_per_launch": tb, "graph_steps": graph_steps},
+                   "steps_per_launch": tb, "graph_steps": graph_steps, "lanes": lanes},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "kernel": "step_multi_kernel", "bytes_per_launch": int(bytes_per_launch),
-                     "launches_per_step": launches, "avg_launch_us": round(avg_launch_s * 1e6, 3)},
+                     "launches_per_step": launches, "avg_launch_us": round(avg_launch_s * 1e6, 3),
+                     "dispatches_per_launch": lanes},
     }
     if tune is not None:
         out["autotune"] = {"schedule": "forward with store + adjoint with imaging, 64 steps",
