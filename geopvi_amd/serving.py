@@ -158,6 +158,8 @@ class _Split:
     syn: Optional[torch.Tensor] = None
     grad: Optional[torch.Tensor] = None
     misfit: Optional[torch.Tensor] = None
+    outputs_ready: Optional[torch.cuda.Event] = None   # recorded after the outputs are zeroed
+    lane: int = 0                                       # the lane whose stream made the outputs
 
 
 @dataclass
@@ -196,15 +198,22 @@ class FWIServer:
     request (ids, times, latency, sizes, store mode).  A job whose misfit
     comes out non-finite (for instance from NaN or Inf in its observed data)
     fails alone with FloatingPointError; the jobs beside it are unaffected.
+    lanes: engines, each on its own stream (1 or 2); a request joins the
+    lane with the fewest units in flight.  With two, one lane's step
+    launches fill the GPU while the other's drain.  A job's results are the
+    same bits on either lane.
     """
 
     def __init__(self, device: int = 0, mem_budget: Optional[int] = None, steps_per_tick: int = 64,
                  max_units: int = 16384, max_tiles: int = 1 << 21, store_policy: str = "auto",
-                 validate_models: bool = True, log_path: Optional[str] = None):
+                 validate_models: bool = True, log_path: Optional[str] = None, lanes: int = 2):
         choose_store(0, 0, 0, 1, store_policy)  # validates the name
+        if lanes not in (1, 2):
+            raise ValueError(f"lanes={lanes}: 1 or 2")
         self.device = torch.device("cuda", device)
         torch.cuda.set_device(self.device)
-        self.stream = torch.cuda.Stream(self.device)
+        self.streams = [torch.cuda.Stream(self.device) for _ in range(lanes)]
+        self.stream = self.streams[0]
         if mem_budget is None:
             free, _ = torch.cuda.mem_get_info(self.device)
             cached = torch.cuda.memory_reserved(self.device) - torch.cuda.memory_allocated(self.device)
@@ -217,10 +226,11 @@ class FWIServer:
         self._log = open(log_path, "a", buffering=1) if log_path else None
         self._next_req = 0
         self._side = torch.cuda.Stream(self.device)  # reads finished outputs without waiting for queued steps
-        self.engine = Engine(self.stream, max_units=max_units, max_tiles=max_tiles)
+        self.engines = [Engine(st, max_units=max_units, max_tiles=max_tiles) for st in self.streams]
+        self.max_units = max_units
         self._queue: collections.deque = collections.deque()
-        self._inflight: dict = {}
-        self._units_in_use = 0
+        self._inflight: dict = {}            # (lane, job id) -> request
+        self._units_in_use = [0] * lanes     # per lane
         self._cv = threading.Condition()
         self._stop = False
         self._error: Optional[BaseException] = None
@@ -305,7 +315,8 @@ class FWIServer:
             self._cv.notify()
         if wait:
             self._thread.join()
-        self.engine.close()
+        for e in self.engines:
+            e.close()
         if self._log is not None:
             self._log.close()
             self._log = None
@@ -344,11 +355,15 @@ class FWIServer:
         off = self.arena.alloc(req.need)
         return None if off is None else (best_ckpt_interval(req.params), req.need, off)
 
+    def _lane_busy(self, lane: int) -> bool:
+        return any(k[0] == lane for k in self._inflight)
+
     def _admit(self):
         while self._queue:
             req = self._queue[0]
             units = req.vel.shape[0] * req.params.ns
-            if self._inflight and self._units_in_use + units > self.engine.max_units:
+            lane = int(np.argmin(self._units_in_use))
+            if self._lane_busy(lane) and self._units_in_use[lane] + units > self.max_units:
                 return
             placed = self._place(req)
             if placed is None:
@@ -361,17 +376,23 @@ class FWIServer:
             ckpt, need, off = placed
             outputs = None
             sp = req.split
+            st = self.streams[lane]
             try:
-                if sp is not None:
-                    if sp.syn is None:  # first chunk: the request's outputs, on the server's stream
-                        n, p = sp.nsample, req.params
-                        sp.syn = torch.empty(n, p.record_len, dtype=torch.float32, device=self.device)
-                        sp.grad = torch.zeros(n, p.ni, dtype=torch.float32, device=self.device)
-                        sp.misfit = torch.empty(n, dtype=torch.float64, device=self.device)
-                    lo, hi = req.rows
-                    outputs = (sp.syn[lo:hi], sp.grad[lo:hi], sp.misfit[lo:hi])
-                job = EngineJob(req.params, req.vel, req.obs, ckpt, req.run_fwi,
-                                workspace=self.arena.view(off, need), outputs=outputs)
+                # outputs are made (and the gradient zeroed) on the lane's stream
+                with torch.cuda.stream(st):
+                    if sp is not None:
+                        if sp.syn is None:  # first chunk: the request's outputs
+                            n, p = sp.nsample, req.params
+                            sp.syn = torch.empty(n, p.record_len, dtype=torch.float32, device=self.device)
+                            sp.grad = torch.zeros(n, p.ni, dtype=torch.float32, device=self.device)
+                            sp.misfit = torch.empty(n, dtype=torch.float64, device=self.device)
+                            sp.outputs_ready = torch.cuda.Event()
+                            sp.outputs_ready.record(st)
+                            sp.lane = lane
+                        lo, hi = req.rows
+                        outputs = (sp.syn[lo:hi], sp.grad[lo:hi], sp.misfit[lo:hi])
+                    job = EngineJob(req.params, req.vel, req.obs, ckpt, req.run_fwi,
+                                    workspace=self.arena.view(off, need), outputs=outputs)
             except torch.OutOfMemoryError as exc:  # the outputs (outside the arena)
                 self.arena.release(off, need)
                 if self._inflight:
@@ -379,9 +400,11 @@ class FWIServer:
                 # nothing running will free memory
                 self._fail_queued(req, MemoryError(f"no room for the outputs: {exc}"))
                 continue
-            self.stream.wait_event(req.ready)
+            st.wait_event(req.ready)
+            if sp is not None and sp.lane != lane:
+                st.wait_event(sp.outputs_ready)
             try:
-                self.engine.submit(job)
+                self.engines[lane].submit(job)
             except _lib.GpviError as exc:
                 if exc.code != _lib.GPVI_ERR_CAPACITY:
                     raise
@@ -400,8 +423,9 @@ class FWIServer:
             req.admitted = time.perf_counter()
             if req.split is not None and not req.split.admitted:
                 req.split.admitted = req.admitted
-            self._inflight[job.id] = req
-            self._units_in_use += units
+            req.extra["lane"] = lane
+            self._inflight[(lane, job.id)] = req
+            self._units_in_use[lane] += units
 
     def _fail_queued(self, req, exc):
         """Drop the request at the head of the queue (with all its queued
@@ -439,11 +463,11 @@ class FWIServer:
         self._log.write(json.dumps(rec) + "\n")
 
     def _reap(self):
-        done = [jid for jid in self._inflight if self.engine.status(jid) == 1]
-        for jid in done:
-            req = self._inflight.pop(jid)
-            self.engine.release(jid)
-            self._units_in_use -= req.vel.shape[0] * req.params.ns
+        done = [k for k in self._inflight if self.engines[k[0]].status(k[1]) == 1]
+        for k in done:
+            req = self._inflight.pop(k)
+            self.engines[k[0]].release(k[1])
+            self._units_in_use[k[0]] -= req.vel.shape[0] * req.params.ns
             job = req.job
             job.workspace = None
             # the job's last kernels are complete (status 1), so its arena
@@ -465,6 +489,7 @@ class FWIServer:
                     continue
                 res = FWIResult(job.misfit, job.grad, job.syn, req.submitted, req.admitted, time.perf_counter())
                 res.extra["ckpt_interval"] = req.ckpt
+                res.extra["lane"] = req.extra["lane"]
             else:
                 sp.remaining -= 1
                 sp.ckpts.append(req.ckpt)
@@ -473,6 +498,7 @@ class FWIServer:
                 res = FWIResult(sp.misfit, sp.grad, sp.syn, sp.submitted, sp.admitted, time.perf_counter())
                 res.extra["ckpt_interval"] = sp.ckpts[0] if len(set(sp.ckpts)) == 1 else list(sp.ckpts)
                 res.extra["chunks"] = len(sp.ckpts)
+                res.extra["lane"] = sp.lane   # the first chunk's
             self.completed.append(res)
             req.future.set_result(res)
             self._write_log(req, res, "ok")
@@ -489,13 +515,21 @@ class FWIServer:
                             break
                         self._admit()
                     if self._inflight:
-                        self.engine.advance(self.steps_per_tick)
+                        busy = [lane for lane in range(len(self.engines)) if self._lane_busy(lane)]
+                        for lane in busy:
+                            self.engines[lane].advance(self.steps_per_tick)
                         n_before = len(self._inflight)
                         self._reap()
-                        if len(self._inflight) == n_before and self.engine.pending_steps() == 0:
+                        if len(self._inflight) == n_before and all(
+                                self.engines[lane].pending_steps() == 0 for lane in busy):
                             # every active job has all its steps queued: wait
-                            # for the device instead of polling the events
-                            self.engine.sync()
+                            # for the device (one lane) instead of polling the
+                            # events, or poll at a short interval (two lanes:
+                            # either may finish first)
+                            if len(busy) == 1:
+                                self.engines[busy[0]].sync()
+                            else:
+                                time.sleep(2e-4)
         except BaseException as exc:  # surface the failure to every waiter
             self._error = exc
             for req in list(self._queue) + list(self._inflight.values()):

@@ -106,12 +106,13 @@ def test_cancel_leaves_others_untouched(three_jobs):
     eng.close()
 
 
-def test_server_concurrent_requests(three_jobs):
+@pytest.mark.parametrize("lanes", [1, 2])
+def test_server_concurrent_requests(three_jobs, lanes):
     from geopvi_amd.serving import FWIServer
     refs = [solver.fwi_batch(p, v, o) for (p, v, o) in three_jobs]
     torch.cuda.synchronize()
     results = [None] * 6
-    with FWIServer(steps_per_tick=32) as srv:
+    with FWIServer(steps_per_tick=32, lanes=lanes) as srv:
         def client(i):
             p, v, o = three_jobs[i % 3]
             results[i] = srv.submit(p, v, o, ckpt_interval=0).result(timeout=120)

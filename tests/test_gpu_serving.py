@@ -82,6 +82,8 @@ def test_serving_mix_equals_standalone_batches(mix, oracle_mod, tmp_path):
         assert torch.equal(r.record_syn, syn) and torch.equal(r.grad, grad) and torch.equal(r.misfit, mis)
         if (p.nz, p.nx) == (400, 800):
             assert r.extra.get("chunks", 1) > 1
+    # requests submitted back to back (each runs for seconds) spread over both lanes
+    assert {r.extra["lane"] for r in res} == {0, 1}
     lines = [json.loads(x) for x in log.read_text().splitlines()]
     assert sorted(x["status"] for x in lines) == ["ok"] * len(mix)
 
