@@ -212,6 +212,10 @@ class FWIServer:
             raise ValueError(f"lanes={lanes}: 1 or 2")
         self.device = torch.device("cuda", device)
         torch.cuda.set_device(self.device)
+        # The side stream first: with few hardware queues the runtime gives a
+        # new stream a least-used queue, and two lane streams made back to
+        # back were seen to share one (then they run one after the other).
+        self._side = torch.cuda.Stream(self.device)  # reads finished outputs without waiting for queued steps
         self.streams = [torch.cuda.Stream(self.device) for _ in range(lanes)]
         self.stream = self.streams[0]
         if mem_budget is None:
@@ -225,7 +229,6 @@ class FWIServer:
         self.steps_per_tick = int(steps_per_tick)
         self._log = open(log_path, "a", buffering=1) if log_path else None
         self._next_req = 0
-        self._side = torch.cuda.Stream(self.device)  # reads finished outputs without waiting for queued steps
         self.engines = [Engine(st, max_units=max_units, max_tiles=max_tiles) for st in self.streams]
         self.max_units = max_units
         self._queue: collections.deque = collections.deque()
