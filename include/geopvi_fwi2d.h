@@ -146,15 +146,16 @@ int gpvi_fwi2d_tile_shape(int variant, int *tile_x, int *tile_z);
  * second computed from the first in registers/LDS (temporal blocking). */
 int gpvi_fwi2d_set_time_blocking(int steps_per_launch);
 int gpvi_fwi2d_get_time_blocking(void);
-/* Lanes of a batched call (1 or 2; default 2).  With 2, the (sample, shot)
- * units are split into two halves (by samples, or by shots for one sample),
- * each advanced by its own step launches on its own stream: the caller's
- * stream and a companion stream the library creates once per caller stream
- * (default priority), joined back to the caller's stream by events.  The
- * step launches of one half then fill the GPU while those of the other
- * drain.  Results are the same bits for 1 and 2.  Process-wide; 1 is
- * always used with GPVI_FLAG_ONE_STREAM, for the pseudo-spectral path, for
- * a single unit and by gpvi_fwi2d_batch_static. */
+/* Lanes of a batched call (1 to 4; default 2).  With n > 1, the (sample,
+ * shot) units are split into n groups of samples (a single sample: its two
+ * shot halves), each advanced by its own step launches on its own stream:
+ * the caller's stream and companion streams the library creates once per
+ * caller stream (default priority), joined back to the caller's stream by
+ * events.  The step launches of one group then fill the GPU while those of
+ * another drain.  Streams only overlap on hardware queues of their own
+ * (GPU_MAX_HW_QUEUES).  Results are the same bits for every n.
+ * Process-wide; one lane is always used with GPVI_FLAG_ONE_STREAM, for the
+ * pseudo-spectral path, for a single unit and by gpvi_fwi2d_batch_static. */
 int gpvi_fwi2d_set_lanes(int lanes);
 int gpvi_fwi2d_get_lanes(void);
 
