@@ -10,7 +10,7 @@ on the GPU, and ``log_prob`` keeps CUDA inputs on the device end to end.
 """
 from __future__ import annotations
 
-from typing import Optional
+from typing import Optional, Union
 
 import numpy as np
 import torch
@@ -52,12 +52,17 @@ class Posterior:
     Extra keyword-only arguments (defaults keep the reference behaviour):
     device: GPU to run on (default: the current one)
     ckpt_interval: 0 = keep every nt_interval-th wavefield (the reference's
-        store); > 0 = checkpoint every this many steps and recompute
+        store); > 0 = checkpoint every this many steps and recompute;
+        "auto" = per call, the full store when its workspace takes at most
+        3/4 of the device memory free to this process (torch's cached
+        blocks counted), else the interval that minimises memory (the
+        results are the same bits either way; the full store saves the
+        recompute pass)
     max_batch: most samples per batched solve (default: all at once)
     """
 
     def __init__(self, data, config, vel_fixed=1950, sigma=0.1, num_processes=1, log_prior=None, mask=None,
-                 paramfile="input_params.txt", *, device=None, ckpt_interval: int = 0,
+                 paramfile="input_params.txt", *, device=None, ckpt_interval: Union[int, str] = 0,
                  max_batch: Optional[int] = None):
         self.data = data
         self.log_prior = log_prior
@@ -77,7 +82,10 @@ class Posterior:
             raise RuntimeError("geopvi_amd: no HIP device visible (the FWI-2D solver runs on the GPU only)")
         self.device = torch.device(device) if device is not None else torch.device(
             "cuda", torch.cuda.current_device())
-        self.ckpt_interval = int(ckpt_interval)
+        if ckpt_interval != "auto":
+            ckpt_interval = int(ckpt_interval)
+        self.ckpt_interval = ckpt_interval
+        self.last_ckpt_interval: Optional[int] = None   # the interval the last solve used
         self.max_batch = max_batch
         self._data_dev = torch.as_tensor(np.ascontiguousarray(data, dtype=np.float32).reshape(-1),
                                          device=self.device)
@@ -112,7 +120,7 @@ class Posterior:
         for lo in range(0, m, step):
             hi = min(m, lo + step)
             _, g, mis = solver.fwi_batch(self.params, v32[lo:hi].contiguous(), self._data_dev,
-                                         ckpt_interval=self.ckpt_interval)
+                                         ckpt_interval=self._ckpt(hi - lo))
             loss[lo:hi] = mis.cpu().numpy()
             grad[lo:hi] = -(g.cpu().numpy()[:, self.mask])
         return loss, grad
@@ -132,6 +140,19 @@ class Posterior:
         logp = -loss / self.sigma ** 2 + self.log_prior(x)
         return logp
 
+    def _ckpt(self, nsample: int) -> int:
+        """The checkpoint interval of a solve of ``nsample`` models."""
+        if self.ckpt_interval != "auto":
+            ck = self.ckpt_interval
+        else:
+            free, _ = torch.cuda.mem_get_info(self.device)
+            free += torch.cuda.memory_reserved(self.device) - torch.cuda.memory_allocated(self.device)
+            full = solver.batch_workspace_bytes(self.params, nsample, 0)
+            from ...engine import best_ckpt_interval
+            ck = 0 if full <= 0.75 * free else best_ckpt_interval(self.params)
+        self.last_ckpt_interval = ck
+        return ck
+
     # -- device-resident path ------------------------------------------------
     def _misfit_device(self, x: torch.Tensor) -> torch.Tensor:
         m = x.shape[0]
@@ -140,6 +161,7 @@ class Posterior:
         outs = []
         step = self.max_batch or m
         for lo in range(0, m, step):
-            outs.append(solver.fwi2d_misfit(vel[lo:lo + step], self._data_dev.to(x.device), self.params,
-                                            self.ckpt_interval))
+            chunk = vel[lo:lo + step]
+            outs.append(solver.fwi2d_misfit(chunk, self._data_dev.to(x.device), self.params,
+                                            self._ckpt(chunk.shape[0])))
         return torch.cat(outs) if len(outs) > 1 else outs[0]

@@ -106,6 +106,25 @@ def test_posterior_host_and_device_paths(case, oracle_mod):
     assert rel_err(loss, ref_loss[0]) <= MISFIT_RTOL and rel_l2(g, ref_grad[0]) <= GRAD_RTOL
 
 
+def test_posterior_store_policy_auto(case):
+    """ckpt_interval="auto" picks the full store for a small problem and gives
+    the bits of the explicit settings (full store and checkpoints)."""
+    p, vt, vs, obs, syn_o, grad_o, path, cfg = case
+    zero = lambda x: torch.zeros(x.shape[0], dtype=x.dtype, device=x.device)
+    rng = np.random.default_rng(3)
+    xs = np.stack([vs.reshape(-1) + 20 * rng.standard_normal(p.ni) for _ in range(2)])
+    out = {}
+    for ck in ("auto", 0, 40):
+        post = Posterior(obs, cfg, sigma=0.5, log_prior=zero, paramfile=path, ckpt_interval=ck)
+        x = torch.tensor(xs, dtype=torch.float64, device="cuda", requires_grad=True)
+        lp = post.log_prob(x)
+        lp.sum().backward()
+        out[ck] = (lp.detach().cpu().numpy(), x.grad.cpu().numpy(), post.last_ckpt_interval)
+    assert out["auto"][2] == 0 and out[40][2] == 40
+    for ck in (0, 40):
+        assert np.array_equal(out["auto"][0], out[ck][0]) and np.array_equal(out["auto"][1], out[ck][1])
+
+
 @pytest.mark.parametrize("ckpt", [0, 50])
 def test_autograd_function_matches_oracle(case, ckpt):
     p, vt, vs, obs, syn_o, grad_o, path, _ = case
