@@ -63,8 +63,10 @@ struct TraceRange {
 
 constexpr int kMaxLanes = 4;
 
+constexpr int kDefaultVariant = 4;  // 24-row strip segments: the fastest on full batches
+
 struct Tuning {
-    int variant = 4;       // 24-row strip segments: fastest measured on the 20-shot 300x100 batch
+    int variant = -1;      // -1: per call (auto_variant); else a fixed tile variant
     int grid = 0;          // 0: automatic
     int graph_steps = 32;  // steps per graph segment
     int tb = 1;            // time steps per multi-unit launch (1 or 2)
@@ -366,7 +368,9 @@ void batch_layout(const Geom& g, const Sched& sc, int nsample, bool tables, char
     L->ktab = g.lc == 0 ? b.take<float>((size_t)ntp) : nullptr;
     if (tables) {
         L->units = b.take<UnitDesc>((size_t)U);
-        L->tiles = b.take<TileRef>((size_t)(U * g.tpu));
+        // sized for the shortest segments (8 rows): the workspace layout is
+        // then the same for every tile variant
+        L->tiles = b.take<TileRef>((size_t)(U * (2 + g.tiles_x) * ((g.ntz + 7) / 8)));
         L->ctl = b.take<StepCtl>(kMaxLanes);   // one per lane
     } else {
         L->units = nullptr; L->tiles = nullptr; L->ctl = nullptr;
@@ -458,6 +462,27 @@ int step_capacity(int lc, int tb)
 // absorbing bands, receiver rows, idle units).  nitems <= 0 (a graph that
 // must serve any item count): twice the resident grid, each wave walking
 // items until the list is done.
+// Tile variant of a batched call when none is set: the longest strip
+// segments (24, 16, then 8 rows) that still give each lane's launch at least
+// one resident grid of work items.  Shorter segments mean more, smaller items,
+// so a small batch fills the GPU; on full batches the 24-row segments are the
+// fastest.  c1, two lanes: one sample 10.5 -> 16.3 samples/s with 8 rows,
+// two 18.6 -> 23.1, four 28.4 -> 29.5; eight and more keep 24 rows.
+int auto_variant(const gpvi_fwi2d_params* p, int nsample, const Tuning& tn)
+{
+    const int units = nsample * p->ns;
+    const int lanes = (tn.lanes > 1 && units >= 2) ? (nsample >= 2 ? std::min(tn.lanes, nsample) : 2) : 1;
+    const int per_lane = (units + lanes - 1) / lanes;
+    static const int order[3] = {4, 1, 3};   // 24, 16, 8 rows per segment
+    for (int v : order) {
+        Geom g;
+        make_geom(p, v, tn.tb, &g);
+        if (g.lc == 0) return kDefaultVariant;   // pseudo-spectral: no strip items
+        if ((int64_t)per_lane * g.tpu >= 4 * (int64_t)step_capacity(g.lc, g.tb)) return v;
+    }
+    return 3;
+}
+
 int launch_grid(int tuned, int nitems, int lc, int tb)
 {
     if (tuned > 0) return std::max(8, tuned / 8 * 8);
@@ -667,7 +692,7 @@ int gpvi_fwi2d_validate(const gpvi_fwi2d_params* p) { return validate(p); }
 
 int gpvi_fwi2d_set_tuning(int variant, int grid, int graph_steps)
 {
-    if (variant < 0 || variant >= kNumVariants) return fail(GPVI_ERR_INVALID, "variant %d out of range", variant);
+    if (variant < -1 || variant >= kNumVariants) return fail(GPVI_ERR_INVALID, "variant %d out of range", variant);
     if (grid < 0 || graph_steps < 1 || graph_steps > 4096) return fail(GPVI_ERR_INVALID, "bad grid/graph_steps");
     std::lock_guard<std::mutex> lk(g_tuning_mu);
     g_tuning.variant = variant;
@@ -825,7 +850,7 @@ size_t gpvi_fwi2d_batch_workspace_bytes(const gpvi_fwi2d_params* p, int nsample,
 {
     if (validate(p) || nsample < 1) return 0;
     Geom g;
-    make_geom(p, tuning().variant, tuning().tb, &g);
+    make_geom(p, kDefaultVariant, tuning().tb, &g);   // the layout is the same for every variant
     Sched sc = make_sched(g, ckpt_interval);
     BatchLayout L;
     batch_layout(g, sc, nsample, true, nullptr, &L);
@@ -836,7 +861,7 @@ size_t gpvi_fwi2d_model_workspace_bytes(const gpvi_fwi2d_params* p, int nsample)
 {
     if (validate(p) || nsample < 1) return 0;
     Geom g;
-    make_geom(p, tuning().variant, tuning().tb, &g);
+    make_geom(p, kDefaultVariant, tuning().tb, &g);
     Sched sc = make_sched(g, 0);
     BatchLayout L;
     batch_layout(g, sc, nsample, true, nullptr, &L, true);
@@ -986,6 +1011,7 @@ int gpvi_fwi2d_batch(const gpvi_fwi2d_params* p, int nsample, const float* vel_i
     rc = phase == GPVI_PHASE_ADJOINT ? layout_stamp_get(workspace, p, nsample, ckpt_interval, &tn)
                                      : 0;
     if (rc) return rc;
+    if (tn.variant < 0) tn.variant = auto_variant(p, nsample, tn);
     Geom g;
     make_geom(p, tn.variant, tn.tb, &g);
     if (g.lc == 0 && (int64_t)nsample * g.ns > 65535)  // one grid row of launches per unit
@@ -1256,7 +1282,7 @@ int gpvi_engine_create(void* stream, int max_units, int max_tiles, gpvi_engine**
     const Tuning t = tuning();
     e->max_units = max_units;
     e->max_tiles = max_tiles;
-    e->variant = t.variant;
+    e->variant = t.variant >= 0 ? t.variant : kDefaultVariant;
     e->tb = t.tb;
     e->seg_steps = t.graph_steps;
     e->grid = t.grid > 0 ? std::max(8, t.grid / 8 * 8) : 0;

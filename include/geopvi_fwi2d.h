@@ -135,9 +135,12 @@ int gpvi_fwi2d_batch_static(const gpvi_fwi2d_params *p, int nsample, const float
                             const float *record_obs, float *record_syn, float *grad, double *misfit, int run_fwi,
                             int ckpt_interval, void *stream);
 
-/* Launch shape of the multi-unit step: tile variant (0..5), persistent grid
- * (a multiple of 8; 0 = automatic) and steps per graph segment.  Process-wide
- * defaults; the autotuner sets them. */
+/* Launch shape of the multi-unit step: tile variant (0..5: rows per strip
+ * segment; -1, the default: chosen per batched call, the longest segments
+ * that still give each lane at least one resident grid of work items),
+ * persistent grid (a multiple of 8; 0 = one wavefront per work item) and
+ * steps per graph segment.  Process-wide; the autotuner sets them.  The
+ * workspace sizes do not depend on the variant. */
 int gpvi_fwi2d_set_tuning(int variant, int grid, int graph_steps);
 int gpvi_fwi2d_get_tuning(int *variant, int *grid, int *graph_steps);
 int gpvi_fwi2d_tile_shape(int variant, int *tile_x, int *tile_z);
