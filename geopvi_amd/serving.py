@@ -527,12 +527,12 @@ class FWIServer:
                                 self.engines[lane].pending_steps() == 0 for lane in busy):
                             # every active job has all its steps queued: wait
                             # for the device (one lane) instead of polling the
-                            # events, or poll at a short interval (two lanes:
-                            # either may finish first)
+                            # events, or poll every 0.5 ms (two lanes: either
+                            # may finish first)
                             if len(busy) == 1:
                                 self.engines[busy[0]].sync()
                             else:
-                                time.sleep(2e-4)
+                                time.sleep(5e-4)
         except BaseException as exc:  # surface the failure to every waiter
             self._error = exc
             for req in list(self._queue) + list(self._inflight.values()):
