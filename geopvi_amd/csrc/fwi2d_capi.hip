@@ -878,24 +878,39 @@ struct PsmPlans {
     hipfftHandle fwd, inv;  // D2Z, Z2D
 };
 std::mutex g_psm_mu;
-std::map<std::tuple<int, int, int, int>, PsmPlans> g_psm_plans;
+std::map<std::tuple<int, int, int, int, hipStream_t>, PsmPlans> g_psm_plans;
 
-int psm_plans(int ntz, int ntx, int batch, PsmPlans* out)
+constexpr size_t kMaxPsmPlans = 16;
+
+// Empty the plan cache (after a device-wide synchronisation) when `fresh`
+// new plan pairs would not fit; called with g_psm_mu held, before the
+// lookups of one call, so that no plan pair a call already holds is destroyed.
+int psm_make_room(size_t fresh)
 {
-    const auto key = std::make_tuple(current_device(), ntz, ntx, batch);
+    if (g_psm_plans.size() + fresh <= kMaxPsmPlans) return 0;
+    HIP_TRY(hipDeviceSynchronize());
+    for (auto& kv : g_psm_plans) {
+        (void)hipfftDestroy(kv.second.fwd);
+        (void)hipfftDestroy(kv.second.inv);
+    }
+    g_psm_plans.clear();
+    return 0;
+}
+
+// The plan pair of a shape, batch and stream, made on first use and bound to
+// that stream.  One pair per stream: executions on two streams may overlap,
+// and a plan's work area must not be shared by them.  Called with g_psm_mu
+// held.
+int psm_plans(int ntz, int ntx, int batch, hipStream_t st, PsmPlans* out)
+{
+    const auto key = std::make_tuple(current_device(), ntz, ntx, batch, st);
     auto it = g_psm_plans.find(key);
     if (it != g_psm_plans.end()) {
         *out = it->second;
         return 0;
     }
-    if (g_psm_plans.size() >= 16) {
-        HIP_TRY(hipDeviceSynchronize());
-        for (auto& kv : g_psm_plans) {
-            (void)hipfftDestroy(kv.second.fwd);
-            (void)hipfftDestroy(kv.second.inv);
-        }
-        g_psm_plans.clear();
-    }
+    if (g_psm_plans.size() >= kMaxPsmPlans)
+        return fail(GPVI_ERR_HIP, "transform plan cache full (make room before the lookups)");
     const int nc = ntx / 2 + 1, rp = 2 * nc;
     int n[2] = {ntz, ntx};
     int real_embed[2] = {ntz, rp}, spec_embed[2] = {ntz, nc};
@@ -907,6 +922,11 @@ int psm_plans(int ntz, int ntx, int batch, PsmPlans* out)
         HIPFFT_SUCCESS) {
         (void)hipfftDestroy(pl.fwd);
         return fail(GPVI_ERR_HIP, "hipfftPlanMany(Z2D %d x %d, batch %d) failed", ntz, ntx, batch);
+    }
+    if (hipfftSetStream(pl.fwd, st) != HIPFFT_SUCCESS || hipfftSetStream(pl.inv, st) != HIPFFT_SUCCESS) {
+        (void)hipfftDestroy(pl.fwd);
+        (void)hipfftDestroy(pl.inv);
+        return fail(GPVI_ERR_HIP, "hipfftSetStream failed");
     }
     g_psm_plans[key] = pl;
     *out = pl;
@@ -955,18 +975,24 @@ int layout_stamp_get(const void* ws, const gpvi_fwi2d_params* p, int nsample, in
     return 0;
 }
 
-int psm_run(hipStream_t st, const Geom& g, const Sched& sc, const JobDesc& j, const BatchLayout& L, int nunit)
+// The pseudo-spectral steps of a batch, lane by lane: lane l advances units
+// [u0[l], u0[l] + un[l]) on stream st[l] (lanes 1.. forked from st[0] by
+// the caller), with its own plan pair over its part of the transform buffer.
+int psm_run(const hipStream_t* st, const int* u0, const int* un, int nlane, const Geom& g, const Sched& sc,
+            const JobDesc& j, const BatchLayout& L)
 {
-    HIP_TRY(launch_psm_wavenumber(st, L.ktab, g.ntx, g.ntz, j.dx, j.dz));
-    // A plan holds one stream: the lock keeps two callers from sharing a
-    // cached plan at once (their batches run one after the other).
-    std::lock_guard<std::mutex> lk(g_psm_mu);
-    PsmPlans pl;
-    int rc;
-    if ((rc = psm_plans(g.ntz, g.ntx, nunit, &pl))) return rc;
-    if (hipfftSetStream(pl.fwd, st) != HIPFFT_SUCCESS || hipfftSetStream(pl.inv, st) != HIPFFT_SUCCESS)
-        return fail(GPVI_ERR_HIP, "hipfftSetStream failed");
-    auto* spec = reinterpret_cast<hipfftDoubleComplex*>(L.fft);
+    PsmPlans pl[kMaxLanes];
+    {
+        std::lock_guard<std::mutex> lk(g_psm_mu);
+        size_t fresh = 0;
+        for (int l = 0; l < nlane; ++l)
+            fresh += g_psm_plans.count(std::make_tuple(current_device(), g.ntz, g.ntx, un[l], st[l])) ? 0 : 1;
+        int rc = psm_make_room(fresh);
+        if (rc) return rc;
+        for (int l = 0; l < nlane; ++l)
+            if ((rc = psm_plans(g.ntz, g.ntx, un[l], st[l], &pl[l]))) return rc;
+    }
+    const int64_t ustride = (int64_t)g.ntz * 2 * (g.ntx / 2 + 1);   // doubles per unit in the buffer
     // Every unit of a batch follows one schedule.  Within a pass the point
     // kernel leaves each step's result in the transform buffer as the next
     // step's input; the first step of a pass (forward, a segment's
@@ -977,14 +1003,19 @@ int psm_run(hipStream_t st, const Geom& g, const Sched& sc, const JobDesc& j, co
     int prev_mode = -1;
     for (int s = 0; s < j.total; ++s) {
         const int mode = decode_step(sd, s + j.s_off).mode;
-        if (mode != prev_mode) HIP_TRY(launch_psm_pack(st, L.units, nunit, s, L.fft, g.ntx, g.ntz));
+        for (int l = 0; l < nlane; ++l) {
+            const UnitDesc* units = L.units + u0[l];
+            double* buf = L.fft + u0[l] * ustride;
+            auto* spec = reinterpret_cast<hipfftDoubleComplex*>(buf);
+            if (mode != prev_mode) HIP_TRY(launch_psm_pack(st[l], units, un[l], s, buf, g.ntx, g.ntz));
+            if (hipfftExecD2Z(pl[l].fwd, buf, spec) != HIPFFT_SUCCESS)
+                return fail(GPVI_ERR_HIP, "hipfftExecD2Z failed");
+            HIP_TRY(launch_psm_scale(st[l], buf, L.ktab, g.ntx, g.ntz, un[l]));
+            if (hipfftExecZ2D(pl[l].inv, spec, buf) != HIPFFT_SUCCESS)
+                return fail(GPVI_ERR_HIP, "hipfftExecZ2D failed");
+            HIP_TRY(launch_psm_point(st[l], units, un[l], s, buf, g.ntx, g.ntz));
+        }
         prev_mode = mode;
-        if (hipfftExecD2Z(pl.fwd, L.fft, spec) != HIPFFT_SUCCESS)
-            return fail(GPVI_ERR_HIP, "hipfftExecD2Z failed");
-        HIP_TRY(launch_psm_scale(st, L.fft, L.ktab, g.ntx, g.ntz, nunit));
-        if (hipfftExecZ2D(pl.inv, spec, L.fft) != HIPFFT_SUCCESS)
-            return fail(GPVI_ERR_HIP, "hipfftExecZ2D failed");
-        HIP_TRY(launch_psm_point(st, L.units, nunit, s, L.fft, g.ntx, g.ntz));
     }
     return 0;
 }
@@ -1036,7 +1067,7 @@ int gpvi_fwi2d_batch(const gpvi_fwi2d_params* p, int nsample, const float* vel_i
     int nlane = 1;
     int lane_t0[kMaxLanes] = {0}, lane_n[kMaxLanes] = {nsample * g.ns * per_unit};
     hipStream_t lane_st[kMaxLanes] = {st};
-    if (g.lc > 0 && tn.lanes > 1 && !(flags & GPVI_FLAG_ONE_STREAM) && nsample * g.ns >= 2) {
+    if (tn.lanes > 1 && !(flags & GPVI_FLAG_ONE_STREAM) && nsample * g.ns >= 2) {
         const int want = nsample >= 2 ? std::min(tn.lanes, nsample) : 2;
         int have = 1;
         while (have < want) {
@@ -1062,7 +1093,22 @@ int gpvi_fwi2d_batch(const gpvi_fwi2d_params* p, int nsample, const float* vel_i
     }
     HIP_TRY(launch_build_units(st, j, L.units, L.tiles));
     if (g.lc == 0) {
-        if ((rc = psm_run(st, g, sc, j, L, nsample * g.ns))) return rc;
+        // lanes in units (the tile counts of the finite-difference path are
+        // units x tiles per unit)
+        int u0[kMaxLanes], un[kMaxLanes];
+        for (int l = 0; l < nlane; ++l) {
+            u0[l] = lane_t0[l] / per_unit;
+            un[l] = lane_n[l] / per_unit;
+        }
+        HIP_TRY(launch_psm_wavenumber(st, L.ktab, g.ntx, g.ntz, j.dx, j.dz));
+        for (int l = 1; l < nlane; ++l)
+            if ((rc = stream_wait(lane_st[l], st))) return rc;
+        rc = psm_run(lane_st, u0, un, nlane, g, sc, j, L);
+        for (int l = 1; l < nlane; ++l) {
+            const int rj = stream_wait(st, lane_st[l]);
+            if (!rc) rc = rj;
+        }
+        if (rc) return rc;
         if (phase != GPVI_PHASE_FORWARD)
             HIP_TRY(launch_grad_reduce(st, L.gpriv, L.grad_stride, grad, g.ns, g.ni, nsample));
         if (phase != GPVI_PHASE_ADJOINT && misfit)

@@ -157,8 +157,8 @@ int gpvi_fwi2d_get_time_blocking(void);
  * events.  The step launches of one group then fill the GPU while those of
  * another drain.  Streams only overlap on hardware queues of their own
  * (GPU_MAX_HW_QUEUES).  Results are the same bits for every n.
- * Process-wide; one lane is always used with GPVI_FLAG_ONE_STREAM, for the
- * pseudo-spectral path, for a single unit and by gpvi_fwi2d_batch_static. */
+ * Process-wide; one lane is always used with GPVI_FLAG_ONE_STREAM, for a
+ * single unit and by gpvi_fwi2d_batch_static. */
 int gpvi_fwi2d_set_lanes(int lanes);
 int gpvi_fwi2d_get_lanes(void);
 

@@ -86,6 +86,27 @@ def test_batch(psm, ckpt):
     assert mis[1] <= 1e-6 * mis[0]
 
 
+def test_lanes_bit_identical(psm):
+    """One to four lanes (groups of samples, or the shot halves of one
+    sample, each with its own transform plans and stream) give the same bits."""
+    p, vt, vs, obs, _, _ = psm
+    vel3 = dev(np.stack([vs.reshape(-1), vt.reshape(-1), 0.5 * (vs + vt).reshape(-1)])).view(3, -1)
+    saved = _lib.get_lanes()
+    try:
+        for n in (1, 3):
+            ref = None
+            for lanes in (1, 2, 4):
+                _lib.set_lanes(lanes)
+                out = [host(t) for t in solver.fwi_batch(p, vel3[:n].contiguous(), dev(obs), ckpt_interval=50)]
+                if ref is None:
+                    ref = out
+                else:
+                    for a, b in zip(out, ref):
+                        assert np.array_equal(a, b), (n, lanes)
+    finally:
+        _lib.set_lanes(saved)
+
+
 def test_forward_batch_and_split_phases(psm):
     p, vt, vs, obs, syn_o, grad_o = psm
     vel = dev(np.stack([vt.reshape(-1), vs.reshape(-1)])).view(2, -1)
