@@ -163,8 +163,8 @@ def serve_stream(args, rate, nreq, seed, dev, mix="bench", compare_batches=False
     wait = np.array([r.admitted - r.submitted for r in res])
     span = t_end - t0
     n_full = sum(1 for r in res if r.extra.get("ckpt_interval") == 0)
-    per_lane = collections.Counter(r.extra.get("lane", 0) for r in res)
-    n_split = sum(1 for r in res if r.extra.get("chunks", 1) > 1)
+    per_lane = collections.Counter(lane for r in res for lane in r.extra.get("lanes", [0]))
+    n_split = sum(1 for r in res if r.extra.get("split_for_memory"))
     batches = None
     if compare_batches:
         # The same requests as back-to-back batched calls (full store), for
@@ -182,7 +182,7 @@ def serve_stream(args, rate, nreq, seed, dev, mix="bench", compare_batches=False
                    "samples_per_s": round(total_samples / tb, 3)}
     policy = server_kw.get("store_policy", "auto")
     return {"requests": nreq, "rate_req_s": rate, "store": f"{policy}: {n_full} full store, {nreq - n_full} checkpointed",
-            "split_requests": n_split, "requests_per_lane": [per_lane[k] for k in sorted(per_lane)],
+            "split_requests": n_split, "requests_on_each_lane": [per_lane[k] for k in sorted(per_lane)],
             "arena_waits": arena_waits,
             "p50_ms": round(float(np.percentile(lat, 50)) * 1e3, 1),
             "p99_ms": round(float(np.percentile(lat, 99)) * 1e3, 1),

@@ -160,6 +160,8 @@ class _Split:
     misfit: Optional[torch.Tensor] = None
     outputs_ready: Optional[torch.cuda.Event] = None   # recorded after the outputs are zeroed
     lane: int = 0                                       # the lane whose stream made the outputs
+    lanes: set = field(default_factory=set)             # lanes its chunks ran on
+    for_memory: bool = False                            # split because it exceeds half the arena
 
 
 @dataclass
@@ -201,15 +203,19 @@ class FWIServer:
     lanes: engines, each on its own stream (1 or 2); a request joins the
     lane with the fewest units in flight.  With two, one lane's step
     launches fill the GPU while the other's drain.  A job's results are the
-    same bits on either lane.
+    same bits on either lane.  split_lanes: with two lanes, run a request of
+    two or more samples as two chunks of samples (one per lane), so both
+    lanes work on it at once.
     """
 
     def __init__(self, device: int = 0, mem_budget: Optional[int] = None, steps_per_tick: int = 64,
                  max_units: int = 16384, max_tiles: int = 1 << 21, store_policy: str = "auto",
-                 validate_models: bool = True, log_path: Optional[str] = None, lanes: int = 2):
+                 validate_models: bool = True, log_path: Optional[str] = None, lanes: int = 2,
+                 split_lanes: bool = True):
         choose_store(0, 0, 0, 1, store_policy)  # validates the name
         if lanes not in (1, 2):
             raise ValueError(f"lanes={lanes}: 1 or 2")
+        self.split_lanes = bool(split_lanes)
         self.device = torch.device("cuda", device)
         torch.cuda.set_device(self.device)
         # The side stream first: with few hardware queues the runtime gives a
@@ -267,7 +273,8 @@ class FWIServer:
         n = vel.shape[0]
         need = job_workspace_bytes(params, n, ck_plan)
         chunk = n
-        if need > self.arena.size // 2:
+        for_memory = need > self.arena.size // 2
+        if for_memory:
             # Too large for one job next to others: split the samples into
             # chunks whose workspace is at most half the arena (two chunks
             # can then run side by side), each written into its rows.
@@ -275,6 +282,10 @@ class FWIServer:
             if chunk == 0:
                 raise MemoryError(f"one sample needs {job_workspace_bytes(params, 1, ck_plan) / 1e9:.1f} GB of "
                                   f"workspace; the arena has {self.arena.size / 1e9:.1f} GB")
+        elif self.split_lanes and len(self.engines) > 1 and n >= 2:
+            # Two chunks, one per lane: both lanes work on the request at
+            # once (the same rows, the same bits).
+            chunk = (n + 1) // 2
         fut: Future = Future()
         reqs = []
         with self._cv:
@@ -285,7 +296,7 @@ class FWIServer:
             ev.record()  # inputs written on the caller's stream are ready here
             reqs.append(_Request(params, vel, obs, int(run_fwi), ckpt, need, ev, fut, t_sub, extra={"id": rid}))
         else:
-            split = _Split(n, (n + chunk - 1) // chunk, t_sub)
+            split = _Split(n, (n + chunk - 1) // chunk, t_sub, for_memory=for_memory)
             ev = torch.cuda.Event()
             ev.record()  # inputs written on the caller's stream are ready here
             for lo in range(0, n, chunk):
@@ -427,6 +438,8 @@ class FWIServer:
             if req.split is not None and not req.split.admitted:
                 req.split.admitted = req.admitted
             req.extra["lane"] = lane
+            if sp is not None:
+                sp.lanes.add(lane)
             self._inflight[(lane, job.id)] = req
             self._units_in_use[lane] += units
 
@@ -493,6 +506,7 @@ class FWIServer:
                 res = FWIResult(job.misfit, job.grad, job.syn, req.submitted, req.admitted, time.perf_counter())
                 res.extra["ckpt_interval"] = req.ckpt
                 res.extra["lane"] = req.extra["lane"]
+                res.extra["lanes"] = [req.extra["lane"]]
             else:
                 sp.remaining -= 1
                 sp.ckpts.append(req.ckpt)
@@ -502,6 +516,8 @@ class FWIServer:
                 res.extra["ckpt_interval"] = sp.ckpts[0] if len(set(sp.ckpts)) == 1 else list(sp.ckpts)
                 res.extra["chunks"] = len(sp.ckpts)
                 res.extra["lane"] = sp.lane   # the first chunk's
+                res.extra["lanes"] = sorted(sp.lanes)
+                res.extra["split_for_memory"] = sp.for_memory
             self.completed.append(res)
             req.future.set_result(res)
             self._write_log(req, res, "ok")

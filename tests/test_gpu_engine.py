@@ -106,13 +106,13 @@ def test_cancel_leaves_others_untouched(three_jobs):
     eng.close()
 
 
-@pytest.mark.parametrize("lanes", [1, 2])
-def test_server_concurrent_requests(three_jobs, lanes):
+@pytest.mark.parametrize("lanes,split", [(1, False), (2, False), (2, True)])
+def test_server_concurrent_requests(three_jobs, lanes, split):
     from geopvi_amd.serving import FWIServer
     refs = [solver.fwi_batch(p, v, o) for (p, v, o) in three_jobs]
     torch.cuda.synchronize()
     results = [None] * 6
-    with FWIServer(steps_per_tick=32, lanes=lanes) as srv:
+    with FWIServer(steps_per_tick=32, lanes=lanes, split_lanes=split) as srv:
         def client(i):
             p, v, o = three_jobs[i % 3]
             results[i] = srv.submit(p, v, o, ckpt_interval=0).result(timeout=120)
@@ -123,6 +123,8 @@ def test_server_concurrent_requests(three_jobs, lanes):
             t.join()
         st = srv.stats()
     assert st["completed"] == 6 and st["p99_s"] >= st["p50_s"] > 0
+    if split:   # every request has 2 or more samples: two chunks, one per lane
+        assert all(r.extra["chunks"] == 2 and r.extra["lanes"] == [0, 1] for r in results)
     for i, r in enumerate(results):
         syn, grad, mis = refs[i % 3]
         assert torch.equal(r.record_syn, syn) and torch.equal(r.grad, grad) and torch.equal(r.misfit, mis)
@@ -138,7 +140,8 @@ def test_server_store_policy(three_jobs, budget_gb, expect_full):
     refs = [solver.fwi_batch(p, v, o) for (p, v, o) in three_jobs]
     torch.cuda.synchronize()
     budget = None if budget_gb is None else int(budget_gb * (1 << 30))
-    with FWIServer(steps_per_tick=32, mem_budget=budget) as srv:
+    # whole requests (no split across the lanes): the choice is per request here
+    with FWIServer(steps_per_tick=32, mem_budget=budget, split_lanes=False) as srv:
         futs = [srv.submit(p, v, o) for (p, v, o) in three_jobs]
         results = [f.result(timeout=120) for f in futs]
     for (p, _, _), r, (syn, grad, mis) in zip(three_jobs, results, refs):
@@ -198,7 +201,8 @@ def test_server_refuses_a_request_beyond_the_engine_tables(three_jobs):
     syn, grad, mis = (t.clone() for t in solver.fwi_batch(p, v1, o))
     torch.cuda.synchronize()
     base = torch.cuda.memory_allocated()
-    with FWIServer(steps_per_tick=32, max_units=units - 1) as srv:
+    # whole requests: split across the lanes, this one would fit
+    with FWIServer(steps_per_tick=32, max_units=units - 1, split_lanes=False) as srv:
         with pytest.raises(_lib.GpviError) as info:
             srv.submit(p, v, o).result(timeout=120)
         assert info.value.code == _lib.GPVI_ERR_CAPACITY
