@@ -280,7 +280,7 @@ void make_geom(const gpvi_fwi2d_params* p, int variant, int tb, Geom* g)
     const bool okz = fastdiv_ok(p->dz, &g->rdz);
     const bool okt = fastdiv_ok(p->dt, &g->rdt);
     g->fastdiv = (okx && okz && okt) ? 1 : 0;
-    const char* off = std::getenv("GPVI_NO_FASTDIV");  // forces the IEEE division path (tests)
+    const char* off = std::getenv("GPVI_NO_FASTDIV");  // every wave takes the fallback division (tests)
     if (off && off[0] && off[0] != '0') g->fastdiv = 0;
 }
 
@@ -783,6 +783,7 @@ static UnitDesc plain_unit(const gpvi_fwi2d_params* p, const Geom& g)
     u.band_w = g.band_w; u.band_tz = g.band_tz;
     u.dt = p->dt; u.dx = p->dx; u.dz = p->dz;
     u.rdx = g.rdx; u.rdz = g.rdz; u.rdt = g.rdt; u.fastdiv = g.fastdiv;
+    u.ydx = 1.0 / (double)p->dx; u.ydz = 1.0 / (double)p->dz; u.ydt = 1.0 / (double)p->dt;
     for (int c = 0; c < 8; ++c) u.fd[c] = g.fd[c];
     u.field_elems = g.fe;
     return u;

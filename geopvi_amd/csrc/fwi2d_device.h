@@ -77,6 +77,7 @@ struct UnitDesc {
     int fastdiv, band_tz;                  // 1: the host verified the reciprocal division for dx, dz, dt;
                                            // side-band items per side
     float fd[8];
+    double ydx, ydz, ydt;  // RN(1/dx), RN(1/dz), RN(1/dt) in double: the exact fallback division
     int64_t field_elems;  // stride between field buffers: (ntz + 2*kPadRows) * pitch
     const float* wtab;    // ABC taper w[i] = 1 - i/pml0, i < pml0
     // Field pointers point at row 0, column 0 of a buffer that has kPadRows

@@ -170,6 +170,7 @@ __global__ void build_units_kernel(JobDesc j, UnitDesc* __restrict__ units, Tile
     u.total = j.total; u.run_fwi = j.run_fwi; u.grad_init = 1; u.store = j.store;
     u.dt = j.dt; u.dx = j.dx; u.dz = j.dz; u.rdt = j.rdt;
     u.rdx = j.rdx; u.rdz = j.rdz; u.fastdiv = j.fastdiv; u.band_tz = j.band_tz;
+    u.ydx = 1.0 / (double)j.dx; u.ydz = 1.0 / (double)j.dz; u.ydt = 1.0 / (double)j.dt;
     for (int c = 0; c < 8; ++c) u.fd[c] = j.fd[c];
     u.field_elems = j.fe;
     u.wtab = j.wtab;

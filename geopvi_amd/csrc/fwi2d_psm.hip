@@ -112,7 +112,7 @@ __global__ void psm_point_kernel(const UnitDesc* __restrict__ units, int s, doub
 
     const bool top = iz < pml, bot = iz >= ntz - pml, lef = ix < pml, rig = ix >= ntx - pml;
     if (top || bot || lef || rig) {
-        const float sc = div1_const(v * kc.dt, kc.dx, kc.rdx, kc.hi);
+        const float sc = div1_const(v * kc.dt, kc.dx, kc.rdx, kc.hi, kc.ydx);
         const float t11 = ((2.f - sc) * (1.f - sc)) / 2.f;
         const float t12 = sc * (2.f - sc);
         const float t13 = (sc * (sc - 1.f)) / 2.f;

@@ -42,11 +42,19 @@ __device__ __forceinline__ __attribute__((address_space(1))) T* G(T* p)
 // That holds for 2^-16 <= |b| <= 2^16 and a == 0 or 2^-96 <= |a| <= 2^96.
 // The range test admits a first dividend in [2^-80, 2^80); its quotient, the
 // dividend of the second division of (a/b)/b, then lies in [2^-96, 2^96] (or
-// is 0), so one test of a covers both divisions.  Other lanes take the IEEE
-// division.  The host checks the divisor range and verifies the identity over
-// a full binade and at the ends of the range before it sets UnitDesc::fastdiv.
-// `hi` is 2^80 when the job may use the identity and 0 when it may not (then
-// no value is in range and every lane divides).
+// is 0), so one test of a covers both divisions.  The host checks the divisor
+// range and verifies the identity over a full binade and at the ends of the
+// range before it sets UnitDesc::fastdiv.  `hi` is 2^80 when the job may use
+// the identity and 0 when it may not (then no value is in range).
+//
+// When any lane of a wave is out of range (in practice: dividends in the
+// subnormal tail ahead of a wavefront), the whole wave divides as
+// RN_f(RN_d(a * RN_d(1/b))), which equals the IEEE quotient RN_f(a/b) for
+// every float a and b: the double product is within 2^-52 (relative) of
+// a/b, while a/b, a quotient of two 24-bit significands, is either a float
+// or at least 2^-49 (relative) away from the nearest float rounding
+// boundary, so the two round alike.  (A CPU test compares the two over the
+// subnormals and every significand of several binades.)
 __device__ __forceinline__ bool cdiv_in_range(float a, float hi)
 {
     const float aa = __builtin_fabsf(a);
@@ -61,14 +69,18 @@ __device__ __forceinline__ float div_hi(const UnitDesc& u) { return u.fastdiv ? 
 // stores alias the descriptor.)
 struct PointConsts {
     float dx, dz, dt, rdx, rdz, rdt, hi;
+    double ydx, ydz, ydt;
     int run_fwi, k, ntx, ntz, pml, pitch4;
 };
 
 __device__ __forceinline__ PointConsts point_consts(const UnitDesc& u)
 {
-    return PointConsts{u.dx, u.dz, u.dt, u.rdx, u.rdz, u.rdt, div_hi(u), u.run_fwi, u.k, u.ntx, u.ntz, u.pml,
-                       u.pitch * 4};
+    return PointConsts{u.dx, u.dz, u.dt, u.rdx, u.rdz, u.rdt, div_hi(u), u.ydx, u.ydz, u.ydt, u.run_fwi, u.k,
+                       u.ntx, u.ntz, u.pml, u.pitch * 4};
 }
+
+// RN_f(a/b) as RN_f(RN_d(a * yd)), yd = RN_d(1/b): exact for every float.
+__device__ __forceinline__ float ddiv(float a, double yd) { return (float)((double)a * yd); }
 
 __device__ __forceinline__ float cdiv_fast(float a, float b, float y)
 {
@@ -79,38 +91,35 @@ __device__ __forceinline__ float cdiv_fast(float a, float b, float y)
 
 // The fallback is taken by a wave-wide vote: when every active lane is in
 // range (the common case) the quotient costs one scalar branch; otherwise
-// the whole wave forms the IEEE quotient and each lane picks its own.
+// the whole wave forms the exact double-product quotient.
 __device__ __forceinline__ bool any_lane(bool x) { return __builtin_amdgcn_ballot_w64(x) != 0; }
 
 // (a / b) / b, as the reference writes d2p/dx/dx and (.)/dt/dt.
-__device__ __forceinline__ float div2_const(float a, float b, float y, float hi)
+__device__ __forceinline__ float div2_const(float a, float b, float y, float hi, double yd)
 {
     const float q = cdiv_fast(cdiv_fast(a, b, y), b, y);
     const bool ok = cdiv_in_range(a, hi);
     if (__builtin_expect(!any_lane(!ok), 1)) return q;
-    const float slow = (a / b) / b;
-    return ok ? q : slow;
+    return ddiv(ddiv(a, yd), yd);
 }
 
-__device__ __forceinline__ float div1_const(float a, float b, float y, float hi)
+__device__ __forceinline__ float div1_const(float a, float b, float y, float hi, double yd)
 {
     const float q = cdiv_fast(a, b, y);
     const bool ok = cdiv_in_range(a, hi);
     if (__builtin_expect(!any_lane(!ok), 1)) return q;
-    const float slow = a / b;
-    return ok ? q : slow;
+    return ddiv(a, yd);
 }
 
 // (a / bx) / bx + (c / bz) / bz with one range test for both quotients.
 // Lanes that are not `active` (their result is discarded) do not vote.
-__device__ __forceinline__ float lap_const(float a, float bx, float yx, float c, float bz, float yz, float hi,
-                                           bool active = true)
+__device__ __forceinline__ float lap_const(float a, float bx, float yx, double ydx, float c, float bz, float yz,
+                                           double ydz, float hi, bool active = true)
 {
     const float q = cdiv_fast(cdiv_fast(a, bx, yx), bx, yx) + cdiv_fast(cdiv_fast(c, bz, yz), bz, yz);
     const bool ok = (cdiv_in_range(a, hi) & cdiv_in_range(c, hi)) | !active;
     if (__builtin_expect(!any_lane(!ok), 1)) return q;
-    const float slow = (a / bx) / bx + (c / bz) / bz;
-    return ok ? q : slow;
+    return ddiv(ddiv(a, ydx), ydx) + ddiv(ddiv(c, ydz), ydz);
 }
 
 // Raw buffer access: a 128-bit scalar resource (base address, byte range,
@@ -269,7 +278,7 @@ __device__ __forceinline__ float point_value(const UnitDesc& u, const PointConst
 #pragma unroll
         for (int m = 1; m < LC; ++m) dzz = dzz + fd_weight(LC, m) * (zn[H + m] + zn[H - m]);
         const float vdt = v * kc.dt;
-        const float lap = lap_const(dxx, kc.dx, kc.rdx, dzz, kc.dz, kc.rdz, hi, active);
+        const float lap = lap_const(dxx, kc.dx, kc.rdx, kc.ydx, dzz, kc.dz, kc.rdz, kc.ydz, hi, active);
         val = (2.f * p1c - p0c) + (vdt * vdt) * lap;
     }
     if (!INSIDE) {
@@ -280,7 +289,7 @@ __device__ __forceinline__ float point_value(const UnitDesc& u, const PointConst
         const bool rig = ix >= ntx - pml && ix < ntx - LC;
         if (top || bot || lef || rig) {
             const int pstep = kc.pitch4;
-            const float sc = div1_const(v * kc.dt, kc.dx, kc.rdx, hi);
+            const float sc = div1_const(v * kc.dt, kc.dx, kc.rdx, hi, kc.ydx);
             const float t11 = ((2.f - sc) * (1.f - sc)) / 2.f;
             const float t12 = sc * (2.f - sc);
             const float t13 = (sc * (sc - 1.f)) / 2.f;
@@ -345,7 +354,7 @@ __device__ __forceinline__ void store_point(const UnitDesc& u, const PointConsts
         }
         if (c.has_save && inner) {
             float sv = val;
-            if (kc.run_fwi) sv = div2_const((val - 2.f * p1c) + p0c, dt, kc.rdt, kc.hi);
+            if (kc.run_fwi) sv = div2_const((val - 2.f * p1c) + p0c, dt, kc.rdt, kc.hi, kc.ydt);
             bstore(sv, c.save, gvo, go);
         }
         if (c.mode == kFwd && rec_row) {

@@ -3,14 +3,16 @@
 The kernels form d2p/dx/dx (and /dz/dz, and the stored (.)/dt/dt) as
 (a/b)/b with the Markstein sequence q0 = a*y, r = fma(-q0, b, a),
 q = fma(r, y, q0), y = RN(1/b), for first dividends a == 0 or
-2^-80 <= |a| < 2^80 and divisors 2^-16 <= |b| <= 2^16; other values take the
-IEEE division (fwi2d_step.h).  The same sequence is compiled here for the
-host with C's fmaf and compared with IEEE division for extreme and ordinary
+2^-80 <= |a| < 2^80 and divisors 2^-16 <= |b| <= 2^16 (fwi2d_step.h).  The
+same sequence is compiled here for the host with C's fmaf and compared with
+IEEE division for extreme and ordinary
 divisors, over every significand of the binades at both ends of the
 dividend range (first division) -- which puts the second division's
 dividend at both ends of its range [2^-96, 2^96] -- and, in between, a
 sample.  Below the range the identity is allowed to fail, which is why the
-kernels test the range.
+kernels test the range: a wave with any other value divides with the
+fallback RN_f(RN_d(a * RN_d(1/b))), compared with IEEE division bit for bit
+at the end of this file.
 """
 import ctypes
 import subprocess
@@ -90,3 +92,32 @@ def test_identity_can_fail_below_the_range(checker):
     # divisor: the kernels must test the range (here: some divisor fails)
     fails = sum(sum(checker(b, -126, 31)) for b in DIVISORS)
     assert fails > 0
+
+
+# The kernels' fallback division: RN_f(RN_d(a * RN_d(1/b))) must equal the
+# IEEE quotient RN_f(a/b) bit for bit (signs of zero included) for every float
+# a, so that a wave with any out-of-range lane can use it for all its lanes.
+
+def _ddiv(a32, b):
+    y = 1.0 / np.float64(np.float32(b))
+    return (a32.astype(np.float64) * y).astype(np.float32)
+
+
+def _binade(e, stride=1):
+    m = np.arange(0, 1 << 23, stride, dtype=np.uint32)
+    bits = ((np.uint32(127 + e) << np.uint32(23)) | m) if e > -127 else m   # e = -127: the subnormals
+    return np.concatenate([bits, bits | np.uint32(1 << 31)]).view(np.float32)
+
+
+@pytest.mark.parametrize("b", DIVISORS)
+def test_double_product_fallback_is_exact(b):
+    b32 = np.float32(b)
+    with np.errstate(over="ignore", under="ignore"):
+        for e, stride in ((-127, 1), (-126, 3), (-100, 7), (-80, 7), (0, 1), (79, 7), (127, 3)):
+            a = _binade(e, stride)
+            ref = a / b32
+            got = _ddiv(a, b)
+            assert np.array_equal(ref.view(np.uint32), got.view(np.uint32)), (b, e)
+            ref2 = ref / b32     # the second division of (a/b)/b
+            got2 = _ddiv(got, b)
+            assert np.array_equal(ref2.view(np.uint32), got2.view(np.uint32)), (b, e)

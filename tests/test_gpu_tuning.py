@@ -132,3 +132,18 @@ def test_lanes_phase_split_and_one_stream(small, restore_tuning):
                        ws, vel_t.device)
     assert np.array_equal(host(syn), syn1) and np.array_equal(host(grad), grad1)
     assert np.array_equal(host(mis), mis1)
+
+
+def test_fallback_division_same_bits(small, monkeypatch):
+    """GPVI_NO_FASTDIV sends every wave through the fallback division (the
+    double product RN_f(RN_d(a * RN_d(1/b)))) instead of the reciprocal
+    sequence: the same bits, forward and adjoint, full store and checkpoints."""
+    p, vel, obs, _, _ = small
+    vel_t, obs_t = dev(vel), dev(obs)
+    for ckpt in (0, 60):
+        fast = [host(t) for t in solver.fwi_batch(p, vel_t, obs_t, ckpt_interval=ckpt)]
+        monkeypatch.setenv("GPVI_NO_FASTDIV", "1")
+        slow = [host(t) for t in solver.fwi_batch(p, vel_t, obs_t, ckpt_interval=ckpt)]
+        monkeypatch.delenv("GPVI_NO_FASTDIV")
+        for a, b in zip(fast, slow):
+            assert np.array_equal(a, b), ckpt
