@@ -483,9 +483,15 @@ def main():
     achieved = bytes_per_launch / avg_launch_s / 1e9
     traffic = None
     tj = _recorded_traffic()
+    profile = None
     if (tj and tj.get("tb") == tb and tj.get("variant") == variant and tj.get("ckpt") == args.ckpt
             and tj.get("lanes", 1) == lanes):
         traffic = tj.get("hbm_bytes_per_launch")
+        # the profile on record: the union of the lanes' dispatch intervals per
+        # launch (comparable with avg_launch_us) and one dispatch's average
+        profile = {"union_us_per_launch": tj.get("union_us_per_launch"),
+                   "avg_dispatch_us": tj.get("avg_dispatch_us"),
+                   "file": "profiles/step_multi_kernel_c1_16x20.md"}
 
     out = {
         "metric": METRIC,
@@ -509,7 +515,7 @@ def main():
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "kernel": "step_multi_kernel", "bytes_per_launch": int(bytes_per_launch),
                      "launches_per_step": launches, "avg_launch_us": round(avg_launch_s * 1e6, 3),
-                     "dispatches_per_launch": lanes},
+                     "dispatches_per_launch": lanes, **({"profile_on_record": profile} if profile else {})},
     }
     if tune is not None:
         out["autotune"] = {"schedule": "forward with store + adjoint with imaging, 64 steps",
