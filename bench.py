@@ -348,11 +348,26 @@ def _recorded_traffic():
         return json.load(f)
 
 
+def _radices(n):
+    """The hand-written transforms' passes for length n (radix 4 first, then
+    2, 3, 5, 7, 11, 13), or None when n has another prime factor."""
+    out, m = [], n
+    while m % 4 == 0:
+        out.append(4)
+        m //= 4
+    for r in (2, 3, 5, 7, 11, 13):
+        while m % r == 0:
+            out.append(r)
+            m //= r
+    return out if m == 1 else None
+
+
 def psm_leg(args, dev, p_fd, samples, obs):
     """The pseudo-spectral Laplacian (laplace_slover = 1) on the c1 geometry:
-    `psm_nsample` samples x 20 shots batched (one hipFFT plan over every
-    unit, double-precision transforms), and one sample alone through the
-    plain entry point.  One untimed warm-up evaluation each."""
+    `psm_nsample` samples x 20 shots batched (double-precision transforms of
+    every unit), one sample alone through the plain entry point, and the
+    batch again with hipFFT's transforms (GPVI_PSM_HIPFFT=1) for comparison.
+    One untimed warm-up evaluation each."""
     import torch
     from geopvi_amd.forward.fwi2d import solver
 
@@ -370,13 +385,24 @@ def psm_leg(args, dev, p_fd, samples, obs):
 
     t_batch = timed(lambda: solver.fwi_batch(p, vel, obs))
     t_plain = timed(lambda: solver.fwi_plain(p, vel[0], obs))
+    os.environ["GPVI_PSM_HIPFFT"] = "1"
+    try:
+        t_lib = timed(lambda: solver.fwi_batch(p, vel, obs))
+    finally:
+        del os.environ["GPVI_PSM_HIPFFT"]
     ntx, ntz = p.nx + 2 * p.pml0, p.nz + 2 * p.pml0
+    rx, rz = _radices(ntx), _radices(ntz)
+    own = rx is not None and rz is not None
     out = {"grid_padded": [ntz, ntx], "nsample": n, "shots": p.ns, "nt": p.nt,
            "batched_samples_per_s": round(n / t_batch, 4), "plain_samples_per_s": round(1 / t_plain, 4),
            "speedup_vs_plain": round(t_plain * n / t_batch, 2),
            "us_per_time_step_batched": round(t_batch / (2 * p.nt) * 1e6, 1),
-           "fft": "hipFFT D2Z + Z2D 2-D in place, double, batch = samples x shots, one pair per time step"}
-    log(f"psm: batched {n / t_batch:.3f} samples/s, plain {1 / t_plain:.3f} samples/s")
+           "transforms": (f"hand-written mixed-radix in LDS, double (rows {'x'.join(map(str, rx))}, columns "
+                          f"{'x'.join(map(str, rz))}; three kernels per time step)") if own else
+                         "hipFFT D2Z + Z2D 2-D in place, double, batch = samples x shots, one pair per time step",
+           "hipfft_batched_samples_per_s": round(n / t_lib, 4),
+           "hipfft_us_per_time_step_batched": round(t_lib / (2 * p.nt) * 1e6, 1)}
+    log(f"psm: batched {n / t_batch:.3f} samples/s (hipFFT {n / t_lib:.3f}), plain {1 / t_plain:.3f} samples/s")
     return out
 
 

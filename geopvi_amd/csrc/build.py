@@ -21,7 +21,7 @@ PKG = os.path.dirname(HERE)
 REPO = os.path.dirname(PKG)
 LIB = os.path.join(PKG, "libgeopvi_fwi2d.so")
 SOURCES = ["fwi2d_kernels.hip", "fwi2d_psm.hip", "fwi2d_capi.hip"]
-HEADERS = ["fwi2d_device.h", "fwi2d_step.h", "fwi2d_kernels.h"]
+HEADERS = ["fwi2d_device.h", "fwi2d_step.h", "fwi2d_kernels.h", "fwi2d_spectral.h"]
 SOURCE_FLAGS = {"fwi2d_kernels.hip": ["-mllvm", "-amdgpu-sched-strategy=max-ilp"]}
 ARCH = os.environ.get("GPVI_OFFLOAD_ARCH", "gfx950")
 

@@ -978,10 +978,21 @@ int layout_stamp_get(const void* ws, const gpvi_fwi2d_params* p, int nsample, in
 
 // The pseudo-spectral steps of a batch, lane by lane: lane l advances units
 // [u0[l], u0[l] + un[l]) on stream st[l] (lanes 1.. forked from st[0] by
-// the caller), with its own plan pair over its part of the transform buffer.
+// the caller) over its part of the transform buffer: with the hand-written
+// transforms when the grid allows them (three launches per step), else with
+// its own hipFFT plan pair.
 int psm_run(const hipStream_t* st, const int* u0, const int* un, int nlane, const Geom& g, const Sched& sc,
             const JobDesc& j, const BatchLayout& L)
 {
+    const int64_t ustride = (int64_t)g.ntz * 2 * (g.ntx / 2 + 1);   // doubles per unit in the buffer
+    const char* lib_fft = std::getenv("GPVI_PSM_HIPFFT");  // the library transforms even where not needed (tests)
+    if (psm_spectral_supported(g.ntx, g.ntz) && !(lib_fft && *lib_fft && std::strcmp(lib_fft, "0") != 0)) {
+        for (int s = 0; s < j.total; ++s)
+            for (int l = 0; l < nlane; ++l)
+                HIP_TRY(launch_psm_spectral(st[l], L.units + u0[l], un[l], s, L.fft + u0[l] * ustride, L.ktab,
+                                            g.ntx, g.ntz));
+        return 0;
+    }
     PsmPlans pl[kMaxLanes];
     {
         std::lock_guard<std::mutex> lk(g_psm_mu);
@@ -993,7 +1004,6 @@ int psm_run(const hipStream_t* st, const int* u0, const int* un, int nlane, cons
         for (int l = 0; l < nlane; ++l)
             if ((rc = psm_plans(g.ntz, g.ntx, un[l], st[l], &pl[l]))) return rc;
     }
-    const int64_t ustride = (int64_t)g.ntz * 2 * (g.ntx / 2 + 1);   // doubles per unit in the buffer
     // Every unit of a batch follows one schedule.  Within a pass the point
     // kernel leaves each step's result in the transform buffer as the next
     // step's input; the first step of a pass (forward, a segment's

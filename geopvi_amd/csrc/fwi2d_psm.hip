@@ -1,30 +1,41 @@
 // Pseudo-spectral Laplacian path (laplace_slover = 1) of the FWI-2D solver
 // for gfx950: geopvi/forward/fwi2d/fdtd2D_modelling.c with Lc = 0
 // (pml = pml0, fdtd.c:75-85) and pstd_2d_calculate_p (fdtd.c:783-830) in
-// place of the stencil.  One time step of every unit of a batch:
+// place of the stencil.  One time step of every unit of a batch, when both
+// padded lengths factor into 2, 3, 5, 7, 11 and 13 and fit the LDS tiling
+// (fwi2d_spectral.h):
 //
-//   pack    p1 -> double field                    (fdtd.c:791-799); only at
-//           the first step of a pass: within a pass the point kernel of
-//           the previous step leaves its result there
+//   rows forward    row pairs of p1 -> half spectra    (fdtd.c:791-803)
+//   columns         column transforms, times -|k|^2, back
+//                                                     (fdtd.c:805-817)
+//   rows backward   half spectra -> rows, then per point
+//                   p2 = 2 p1 - p0 + (v dt)^2 Re / (ntx ntz), the four
+//                   ordered absorbing sweeps over the pml0-wide bands, source
+//                   or residual injection, store/records or imaging
+//                                                     (fdtd.c:819-826, 833-978)
+//
+// and otherwise (or with GPVI_PSM_HIPFFT set) through hipFFT:
+//
+//   pack    p1 -> double field; only at the first step of a pass: within a
+//           pass the point kernel of the previous step leaves its result there
 //   FFT     batched 2-D forward transform         (hipFFT, D2Z)
-//   scale   multiply by -|k|^2                    (fdtd.c:805-813)
+//   scale   multiply by -|k|^2
 //   IFFT    batched 2-D inverse (unnormalised)    (hipFFT, Z2D)
-//   point   p2 = 2 p1 - p0 + (v dt)^2 Re / (ntx ntz), then the four ordered
-//           absorbing sweeps over the pml0-wide bands, source or residual
-//           injection, store/records or imaging   (fdtd.c:819-826, 833-978)
+//   point   as the rows-backward kernel's point update
 //
 // The reference transforms the real field as a complex one (fftw c2c) and
 // keeps the real part of the inverse.  The spectrum of a real field is
 // Hermitian and |k|^2 is even in both axes (kx(ix) = kx(ntx - ix)), so the
-// scaled spectrum stays Hermitian and its inverse is real: the
-// real-to-complex / complex-to-real pair computes the same values with half
-// the transform work; only the rounding differs, as between any two FFT
-// libraries.  Transforms are in place, double precision (FFTW's
-// fftw_complex is double); real rows are padded to rp = 2 (ntx/2 + 1)
-// doubles, the spectrum has ntx/2 + 1 complex columns.  Every other
-// operation keeps the reference's float/double typing.
+// scaled spectrum stays Hermitian and its inverse is real: real-to-complex /
+// complex-to-real transforms compute the same values with half the work;
+// only the rounding differs, as between any two FFT libraries.  Transforms
+// are double precision (FFTW's fftw_complex is double); a unit's buffer
+// holds ntz rows of nc = ntx/2 + 1 complex values (for hipFFT, in place:
+// real rows padded to 2 nc doubles).  Every other operation keeps the
+// reference's float/double typing.
 #include "fwi2d_kernels.h"
 #include "fwi2d_step.h"
+#include "fwi2d_spectral.h"
 
 namespace gpvi {
 
@@ -79,36 +90,44 @@ __global__ void psm_scale_kernel(double2* __restrict__ spec, const float* __rest
     }
 }
 
-// Everything after the Laplacian, one thread per padded point of a unit.
-__global__ void psm_point_kernel(const UnitDesc* __restrict__ units, int s, double* __restrict__ re_all,
-                                 int rp, int64_t ustride, int ntp)
+// p1, p0 and v at point (iz, ix) of a unit.
+struct PsmCore {
+    float p1, p0, v;
+};
+__device__ inline unsigned psm_vo(const PointConsts& kc, int iz, int ix, int ro)
 {
-    const int un = blockIdx.y;
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= ntp) return;
-    const UnitDesc& u = units[un];
-    const int ls = s - u.start + u.s_off;  // the unit's local step
-    if (ls < u.s_off || ls >= u.s_off + u.total) return;
-    const StepInfo si = decode_step(u, ls);
-    if (si.mode == kIdle) return;
-    const FieldPtrs f = step_fields(u, si);
+    return (unsigned)((iz + kPadRows) * kc.pitch4 + ix * 4 - ro);
+}
+__device__ inline PsmCore psm_core(const UnitDesc& u, const FieldPtrs& f, const PointConsts& kc, int iz, int ix)
+{
     const int64_t margin = (int64_t)kPadRows * u.pitch;
-    const StepCtx c = make_step(u, si.mode, si.it, si.seg, f.out, margin);
-    const PointConsts kc = point_consts(u);
+    const int ro = 4 * kc.pitch4 + 16;
+    const unsigned vo = psm_vo(kc, iz, ix, ro);
+    PsmCore r;
+    r.p1 = bload(make_rsrc(f.p1 - margin), vo, ro);
+    r.p0 = bload(make_rsrc(f.p0 - margin), vo, ro);
+    r.v = bload(make_rsrc(u.vel - margin), vo, ro);
+    return r;
+}
+
+// Everything after the Laplacian at point (iz, ix) of a unit, from its p1,
+// p0 and v: the update p2 = 2 p1 - p0 + (v dt)^2 re / div (re / div = the
+// Laplacian), the four ordered absorbing blends over the pml0-wide bands,
+// injection, store or imaging and records.  Returns p2.
+__device__ inline float psm_point(const UnitDesc& u, const FieldPtrs& f, const StepCtx& c, const PointConsts& kc,
+                                  int iz, int ix, double re, double div, PsmCore core)
+{
     const int ntx = kc.ntx, ntz = kc.ntz, pml = kc.pml, pitch4 = kc.pitch4;
-    const int iz = i / ntx, ix = i - iz * ntx;
+    const int64_t margin = (int64_t)kPadRows * u.pitch;
     // byte offset of (iz, ix) from the first margin row, split so that the
     // inward reads (up to 4 rows or columns either way) stay >= 0
     const int ro = 4 * pitch4 + 16;
-    const unsigned vo = (unsigned)((iz + kPadRows) * pitch4 + ix * 4 - ro);
+    const unsigned vo = psm_vo(kc, iz, ix, ro);
     const Rsrc rp1 = make_rsrc(f.p1 - margin);
     const Rsrc rp0 = make_rsrc(f.p0 - margin);
-    const Rsrc rv = make_rsrc(u.vel - margin);
-    const float p1c = bload(rp1, vo, ro), p0c = bload(rp0, vo, ro), v = bload(rv, vo, ro);
-    double* const re_pt = re_all + un * ustride + (int64_t)iz * rp + ix;
-    const double re = *re_pt;
+    const float p1c = core.p1, p0c = core.p0, v = core.v;
     const float vdt = v * kc.dt;
-    float val = (float)((double)(2.f * p1c - p0c) + (double)(vdt * vdt) * re / (double)ntp);
+    float val = (float)((double)(2.f * p1c - p0c) + (double)(vdt * vdt) * re / div);
 
     const bool top = iz < pml, bot = iz >= ntz - pml, lef = ix < pml, rig = ix >= ntx - pml;
     if (top || bot || lef || rig) {
@@ -149,7 +168,176 @@ __global__ void psm_point_kernel(const UnitDesc* __restrict__ units, int s, doub
     const bool inner = iz >= pml && iz < ntz - pml && ix >= pml && ix < ntx - pml;
     const unsigned gvo = inner ? (unsigned)(((iz - pml) * u.nx + (ix - pml)) * 4) : 0u;
     store_point(u, kc, c, val, p0c, p1c, v, ix, vo, ro, gvo, 0, make_rsrc(u.grad), rec_row, inner);
+    return val;
+}
+
+// The unit of a block at batch step s and its step (false: idle at s).
+__device__ inline bool psm_unit_step(const UnitDesc& u, int s, StepInfo* si)
+{
+    const int ls = s - u.start + u.s_off;  // the unit's local step
+    if (ls < u.s_off || ls >= u.s_off + u.total) return false;
+    *si = decode_step(u, ls);
+    return si->mode != kIdle;
+}
+
+// After the library transforms: one thread per padded point of a unit.
+__global__ void psm_point_kernel(const UnitDesc* __restrict__ units, int s, double* __restrict__ re_all,
+                                 int rp, int64_t ustride, int ntp)
+{
+    const int un = blockIdx.y;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= ntp) return;
+    const UnitDesc& u = units[un];
+    StepInfo si;
+    if (!psm_unit_step(u, s, &si)) return;
+    const FieldPtrs f = step_fields(u, si);
+    const int64_t margin = (int64_t)kPadRows * u.pitch;
+    const StepCtx c = make_step(u, si.mode, si.it, si.seg, f.out, margin);
+    const PointConsts kc = point_consts(u);
+    const int iz = i / kc.ntx, ix = i - iz * kc.ntx;
+    double* const re_pt = re_all + un * ustride + (int64_t)iz * rp + ix;
+    const float val = psm_point(u, f, c, kc, iz, ix, *re_pt, (double)ntp, psm_core(u, f, kc, iz, ix));
     *re_pt = (double)val;  // the next step's transform input (this thread's own element)
+}
+
+// ---------------------------------------------------------------------------
+// Hand-written transforms (fwi2d_spectral.h): three kernels per step over a
+// per-unit half spectrum of ntz x nc complex doubles (nc = ntx/2 + 1).
+using spec::Cx;
+
+extern __shared__ __align__(16) unsigned char psm_lds[];
+
+// T[m] = (cos, sin)(2 pi m / n).
+__device__ inline void psm_table(Cx* T, int n)
+{
+    for (int m = threadIdx.x; m < n; m += blockDim.x) {
+        double sn, cs;
+        sincospi(2.0 * (double)m / (double)n, &sn, &cs);
+        T[m] = spec::cx(cs, sn);
+    }
+}
+
+// The plan's passes over nseq sequences, x -> y -> x ... (a barrier after
+// each); returns the buffer holding the result.  Expects a barrier before.
+template <int MaxR>
+__device__ inline Cx* psm_transform(Cx* x, Cx* y, int nseq, const spec::Plan& p, const Cx* T, int sgn)
+{
+    int ns = 1;
+    for (int i = 0; i < p.npass; ++i) {
+        spec::run_pass<MaxR>(p.radix[i], x, y, nseq, p.n, ns, T, sgn, threadIdx.x, blockDim.x);
+        __syncthreads();
+        ns *= p.radix[i];
+        Cx* t = x;
+        x = y;
+        y = t;
+    }
+    return x;
+}
+
+// Rows forward: block (g, unit) transforms row pairs [g np, (g + 1) np) of
+// p1 and writes the half spectra of their rows.
+template <int MaxR>
+__global__ void __launch_bounds__(spec::kThreads)
+psm_rows_fwd_kernel(const UnitDesc* __restrict__ units, int s, Cx* __restrict__ spec_all, int64_t ustride,
+                    spec::Plan px, int ntz, int np)
+{
+    const UnitDesc& u = units[blockIdx.y];
+    StepInfo si;
+    if (!psm_unit_step(u, s, &si)) return;  // the whole block
+    const FieldPtrs f = step_fields(u, si);
+    const int ntx = px.n, nc = ntx / 2 + 1, pair0 = blockIdx.x * np;
+    Cx* const sp = spec_all + blockIdx.y * ustride;
+    Cx* T = reinterpret_cast<Cx*>(psm_lds);
+    Cx* X = T + ntx;
+    Cx* Y = X + np * ntx;
+    psm_table(T, ntx);
+    const float* p1 = f.p1;
+    const int pitch = u.pitch;
+    spec::rows_load(X, np, ntx, ntz, pair0, [&](int iz, int ix) { return (double)p1[(int64_t)iz * pitch + ix]; },
+                    threadIdx.x, blockDim.x);
+    __syncthreads();
+    const Cx* Z = psm_transform<MaxR>(X, Y, np, px, T, +1);
+    spec::rows_unpack(Z, np, ntx, ntz, nc, pair0, [&](int iz, int fq, Cx v) { sp[(int64_t)iz * nc + fq] = v; },
+                      threadIdx.x, blockDim.x);
+}
+
+// Columns: block (g, unit) transforms spectrum columns [g ncol, (g + 1) ncol),
+// scales them by -|k|^2 and transforms them back.
+template <int MaxR>
+__global__ void __launch_bounds__(spec::kThreads)
+psm_cols_kernel(const UnitDesc* __restrict__ units, int s, Cx* __restrict__ spec_all, int64_t ustride,
+                const float* __restrict__ k, spec::Plan pz, int ntx, int ncol)
+{
+    const UnitDesc& u = units[blockIdx.y];
+    StepInfo si;
+    if (!psm_unit_step(u, s, &si)) return;  // the whole block
+    const int ntz = pz.n, nc = ntx / 2 + 1, f0 = blockIdx.x * ncol;
+    Cx* const sp = spec_all + blockIdx.y * ustride;
+    Cx* T = reinterpret_cast<Cx*>(psm_lds);
+    Cx* X = T + ntz;
+    Cx* Y = X + ncol * ntz;
+    psm_table(T, ntz);
+    spec::cols_load(X, ncol, ntz, nc, f0, [&](int iz, int fq) { return sp[(int64_t)iz * nc + fq]; }, threadIdx.x,
+                    blockDim.x);
+    __syncthreads();
+    Cx* Z = psm_transform<MaxR>(X, Y, ncol, pz, T, +1);
+    spec::cols_scale(Z, ncol, ntz, nc, f0, [&](int iz, int fq) { return k[(int64_t)iz * ntx + fq]; }, threadIdx.x,
+                     blockDim.x);
+    __syncthreads();
+    const Cx* W = psm_transform<MaxR>(Z, Z == X ? Y : X, ncol, pz, T, -1);
+    spec::cols_store(W, ncol, ntz, nc, f0, [&](int iz, int fq, Cx v) { sp[(int64_t)iz * nc + fq] = v; },
+                     threadIdx.x, blockDim.x);
+}
+
+// Rows backward and the point update: block (g, unit) transforms row pairs
+// [g np, (g + 1) np) of the scaled spectrum back and finishes their points.
+template <int MaxR>
+__global__ void __launch_bounds__(spec::kThreads)
+psm_rows_inv_kernel(const UnitDesc* __restrict__ units, int s, const Cx* __restrict__ spec_all, int64_t ustride,
+                    spec::Plan px, int ntz, int np)
+{
+    const UnitDesc& u = units[blockIdx.y];
+    StepInfo si;
+    if (!psm_unit_step(u, s, &si)) return;  // the whole block
+    const int ntx = px.n, nc = ntx / 2 + 1, pair0 = blockIdx.x * np;
+    const Cx* const sp = spec_all + blockIdx.y * ustride;
+    Cx* T = reinterpret_cast<Cx*>(psm_lds);
+    Cx* X = T + ntx;
+    Cx* Y = X + np * ntx;
+    psm_table(T, ntx);
+    spec::rows_repack(X, np, ntx, ntz, pair0, [&](int iz, int fq) { return sp[(int64_t)iz * nc + fq]; },
+                      threadIdx.x, blockDim.x);
+    __syncthreads();
+    const Cx* Z = psm_transform<MaxR>(X, Y, np, px, T, -1);
+    const FieldPtrs f = step_fields(u, si);
+    const int64_t margin = (int64_t)kPadRows * u.pitch;
+    const StepCtx c = make_step(u, si.mode, si.it, si.seg, f.out, margin);
+    const PointConsts kc = point_consts(u);
+    const double ntp = (double)ntx * (double)ntz;
+    // the points of the block's rows, spec::kChunk per thread at a time with
+    // their p1, p0 and v loads in flight together
+    const int total = 2 * np * ntx;
+    for (int t0 = threadIdx.x; t0 < total; t0 += spec::kChunk * blockDim.x) {
+        PsmCore core[spec::kChunk];
+#pragma unroll
+        for (int j = 0; j < spec::kChunk; ++j) {
+            const int t = t0 + j * blockDim.x;
+            const int q = t / (2 * ntx), r = t - q * 2 * ntx;
+            const int w = r >= ntx ? 1 : 0, ix = r - w * ntx;
+            const int iz = 2 * (pair0 + q) + w;
+            if (t < total && iz < ntz) core[j] = psm_core(u, f, kc, iz, ix);
+        }
+#pragma unroll
+        for (int j = 0; j < spec::kChunk; ++j) {
+            const int t = t0 + j * blockDim.x;
+            const int q = t / (2 * ntx), r = t - q * 2 * ntx;
+            const int w = r >= ntx ? 1 : 0, ix = r - w * ntx;
+            const int iz = 2 * (pair0 + q) + w;
+            if (t >= total || iz >= ntz) continue;
+            const Cx z = Z[q * ntx + ix];
+            psm_point(u, f, c, kc, iz, ix, w ? z.im : z.re, ntp, core[j]);
+        }
+    }
 }
 
 hipError_t launch_psm_wavenumber(hipStream_t st, float* k, int ntx, int ntz, float dx, float dz)
@@ -187,6 +375,58 @@ hipError_t launch_psm_point(hipStream_t st, const UnitDesc* units, int nunit, in
     const int ntp = ntx * ntz, rp = psm_rp(ntx);
     hipLaunchKernelGGL(psm_point_kernel, dim3((ntp + 255) / 256, nunit), dim3(256), 0, st, units, s, buf, rp,
                        (int64_t)ntz * rp, ntp);
+    return hipGetLastError();
+}
+
+static bool psm_spectral_plans(int ntx, int ntz, spec::Plan* px, spec::Plan* pz, int* np, int* ncol)
+{
+    if (!spec::make_plan(ntx, px) || !spec::make_plan(ntz, pz)) return false;
+    *np = spec::pairs_per_group(ntx);
+    *ncol = spec::cols_per_group(ntz);
+    return *np > 0 && *ncol > 0;
+}
+
+bool psm_spectral_supported(int ntx, int ntz)
+{
+    spec::Plan px, pz;
+    int np, ncol;
+    return psm_spectral_plans(ntx, ntz, &px, &pz, &np, &ncol);
+}
+
+hipError_t launch_psm_spectral(hipStream_t st, const UnitDesc* units, int nunit, int s, double* buf,
+                               const float* k, int ntx, int ntz)
+{
+    spec::Plan px, pz;
+    int np, ncol;
+    if (!psm_spectral_plans(ntx, ntz, &px, &pz, &np, &ncol)) return hipErrorInvalidValue;
+    const int nc = ntx / 2 + 1;
+    // the buffer holds ntz * 2 nc doubles per unit = ntz * nc complex values
+    const int64_t ustride = (int64_t)ntz * nc;
+    Cx* sp = reinterpret_cast<Cx*>(buf);
+    const int row_groups = ((ntz + 1) / 2 + np - 1) / np, col_groups = (nc + ncol - 1) / ncol;
+    const size_t lds_rows = sizeof(Cx) * (size_t)ntx * (1 + 2 * np);
+    const size_t lds_cols = sizeof(Cx) * (size_t)ntz * (1 + 2 * ncol);
+    // instantiations by largest radix: 7, 11 or 13
+    const int mx = spec::max_radix(px), mz = spec::max_radix(pz);
+    const dim3 rows(row_groups, nunit), cols(col_groups, nunit), blk(spec::kThreads);
+    if (mx <= 7)
+        hipLaunchKernelGGL(psm_rows_fwd_kernel<7>, rows, blk, lds_rows, st, units, s, sp, ustride, px, ntz, np);
+    else if (mx == 11)
+        hipLaunchKernelGGL(psm_rows_fwd_kernel<11>, rows, blk, lds_rows, st, units, s, sp, ustride, px, ntz, np);
+    else
+        hipLaunchKernelGGL(psm_rows_fwd_kernel<13>, rows, blk, lds_rows, st, units, s, sp, ustride, px, ntz, np);
+    if (mz <= 7)
+        hipLaunchKernelGGL(psm_cols_kernel<7>, cols, blk, lds_cols, st, units, s, sp, ustride, k, pz, ntx, ncol);
+    else if (mz == 11)
+        hipLaunchKernelGGL(psm_cols_kernel<11>, cols, blk, lds_cols, st, units, s, sp, ustride, k, pz, ntx, ncol);
+    else
+        hipLaunchKernelGGL(psm_cols_kernel<13>, cols, blk, lds_cols, st, units, s, sp, ustride, k, pz, ntx, ncol);
+    if (mx <= 7)
+        hipLaunchKernelGGL(psm_rows_inv_kernel<7>, rows, blk, lds_rows, st, units, s, sp, ustride, px, ntz, np);
+    else if (mx == 11)
+        hipLaunchKernelGGL(psm_rows_inv_kernel<11>, rows, blk, lds_rows, st, units, s, sp, ustride, px, ntz, np);
+    else
+        hipLaunchKernelGGL(psm_rows_inv_kernel<13>, rows, blk, lds_rows, st, units, s, sp, ustride, px, ntz, np);
     return hipGetLastError();
 }
 

@@ -1,8 +1,9 @@
 """Pseudo-spectral Laplacian path (laplace_slover = 1) on the GPU against the
 numpy PSM oracle (oracle/psm_oracle.py), on identical seeded inputs.
 
-The transforms are double precision on both sides (hipFFT here, numpy's
-pocketfft in the oracle).  Two FFT libraries agree to double rounding
+The transforms are double precision on both sides (the library's own
+mixed-radix transforms, or hipFFT for lengths they do not cover, here;
+numpy's pocketfft in the oracle).  Two FFT libraries agree to double rounding
 relative to the field's largest value, not per point; rounded to float, the
 Laplacian then differs in the last bit at many points and the records of the
 two libraries drift apart by ~2e-6 relative L2 over a few hundred steps
@@ -131,6 +132,40 @@ def test_aco2d_and_host_entry_points(psm, tmp_path):
     v32 = np.ascontiguousarray(vt.reshape(-1), np.float32)
     lib.forward_2D(path.encode(), v32.ctypes.data_as(fp), rec.ctypes.data_as(fp), 1, 0)
     assert lib.gpvi_compat_last_status() == 0 and rel_l2(rec, obs) <= REC_RTOL
+
+
+def test_library_transforms(psm, monkeypatch):
+    """GPVI_PSM_HIPFFT=1 runs the same batch through hipFFT instead of the
+    hand-written transforms: both agree with the oracle to the bars, and with
+    each other to the transforms' rounding."""
+    p, vt, vs, obs, syn_o, grad_o = psm
+    vel = dev(np.stack([vs.reshape(-1), vt.reshape(-1)])).view(2, -1)
+    own = [host(t) for t in solver.fwi_batch(p, vel, dev(obs), ckpt_interval=50)]
+    monkeypatch.setenv("GPVI_PSM_HIPFFT", "1")
+    lib = [host(t) for t in solver.fwi_batch(p, vel, dev(obs), ckpt_interval=50)]
+    monkeypatch.delenv("GPVI_PSM_HIPFFT")
+    for tag, (syn, grad, mis) in (("own", own), ("hipfft", lib)):
+        _report(f"transforms {tag}", syn[0], syn_o, grad[0], grad_o)
+        assert rel_l2(syn[0], syn_o) <= REC_RTOL
+        assert rel_err(mis[0], misfit64(syn_o, obs)) <= MISFIT_RTOL
+        assert rel_l2(grad[0], grad_o) <= GRAD_RTOL
+    print(f"[psm own vs hipfft] rec {rel_l2(own[0], lib[0]):.2e} grad {rel_l2(own[1], lib[1]):.2e}", flush=True)
+    assert rel_l2(own[0], lib[0]) <= REC_RTOL and rel_l2(own[1], lib[1]) <= GRAD_RTOL
+
+
+def test_length_with_a_large_prime_factor():
+    """ntx = 34 = 2 x 17 has no hand-written radix: the step runs through
+    hipFFT and still meets the bars."""
+    p, vt, vs = case("tiny", seed=7, laplace_slover=1, nx=14, ns=2, ds=8, nr=7, nt=200)
+    assert p.nx + 2 * p.pml0 == 34
+    obs = P.forward(p, vt)
+    syn_o, grad_o = P.fwi(p, vs, obs)
+    syn, grad, mis = (host(t) for t in solver.fwi_batch(p, dev(vs).view(1, -1), dev(obs), ckpt_interval=0))
+    _report("ntx=34", syn[0], syn_o, grad[0], grad_o)
+    assert np.abs(syn_o).max() > 0
+    assert rel_l2(syn[0], syn_o) <= REC_RTOL
+    assert rel_err(mis[0], misfit64(syn_o, obs)) <= MISFIT_RTOL
+    assert rel_l2(grad[0], grad_o) <= GRAD_RTOL
 
 
 def test_engine_rejects_pseudo_spectral(psm):
