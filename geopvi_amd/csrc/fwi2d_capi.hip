@@ -333,6 +333,7 @@ struct BatchLayout {
     double* partial;
     double* fft;    // pseudo-spectral path: per unit, ntz rows of 2 (ntx/2 + 1) doubles (in-place r2c/c2r)
     float* ktab;    // pseudo-spectral path: |k|^2 per padded point
+    double* twid;   // pseudo-spectral path: (cos, sin)(2 pi m / n), m < n, for n = ntx, then for n = ntz
     UnitDesc* units;
     TileRef* tiles;
     StepCtl* ctl;
@@ -366,6 +367,7 @@ void batch_layout(const Geom& g, const Sched& sc, int nsample, bool tables, char
     const int64_t ntp = (int64_t)g.ntx * g.ntz;
     L->fft = g.lc == 0 ? b.take<double>((size_t)(U * g.ntz * 2 * (g.ntx / 2 + 1))) : nullptr;
     L->ktab = g.lc == 0 ? b.take<float>((size_t)ntp) : nullptr;
+    L->twid = g.lc == 0 ? b.take<double>((size_t)2 * (g.ntx + g.ntz)) : nullptr;
     if (tables) {
         L->units = b.take<UnitDesc>((size_t)U);
         // sized for the shortest segments (8 rows): the workspace layout is
@@ -989,7 +991,7 @@ int psm_run(const hipStream_t* st, const int* u0, const int* un, int nlane, cons
     if (psm_spectral_supported(g.ntx, g.ntz) && !(lib_fft && *lib_fft && std::strcmp(lib_fft, "0") != 0)) {
         for (int s = 0; s < j.total; ++s)
             for (int l = 0; l < nlane; ++l)
-                HIP_TRY(launch_psm_spectral(st[l], L.units + u0[l], un[l], s, L.fft + u0[l] * ustride, L.ktab,
+                HIP_TRY(launch_psm_spectral(st[l], L.units + u0[l], un[l], s, L.fft + u0[l] * ustride, L.ktab, L.twid,
                                             g.ntx, g.ntz));
         return 0;
     }
@@ -1112,6 +1114,7 @@ int gpvi_fwi2d_batch(const gpvi_fwi2d_params* p, int nsample, const float* vel_i
             un[l] = lane_n[l] / per_unit;
         }
         HIP_TRY(launch_psm_wavenumber(st, L.ktab, g.ntx, g.ntz, j.dx, j.dz));
+        HIP_TRY(launch_psm_twiddles(st, L.twid, g.ntx, g.ntz));
         for (int l = 1; l < nlane; ++l)
             if ((rc = stream_wait(lane_st[l], st))) return rc;
         rc = psm_run(lane_st, u0, un, nlane, g, sc, j, L);

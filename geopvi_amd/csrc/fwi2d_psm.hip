@@ -207,14 +207,21 @@ using spec::Cx;
 
 extern __shared__ __align__(16) unsigned char psm_lds[];
 
-// T[m] = (cos, sin)(2 pi m / n).
-__device__ inline void psm_table(Cx* T, int n)
+// The twiddle tables of a job, (cos, sin)(2 pi m / n) for m < n, n = ntx
+// then n = ntz; made once per call, copied to LDS by every workgroup.
+__global__ void psm_twiddle_kernel(Cx* __restrict__ tw, int ntx, int ntz)
 {
-    for (int m = threadIdx.x; m < n; m += blockDim.x) {
-        double sn, cs;
-        sincospi(2.0 * (double)m / (double)n, &sn, &cs);
-        T[m] = spec::cx(cs, sn);
-    }
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= ntx + ntz) return;
+    const int n = i < ntx ? ntx : ntz, m = i < ntx ? i : i - ntx;
+    double sn, cs;
+    sincospi(2.0 * (double)m / (double)n, &sn, &cs);
+    tw[i] = spec::cx(cs, sn);
+}
+
+__device__ inline void psm_table(Cx* T, const Cx* __restrict__ tw, int n)
+{
+    for (int m = threadIdx.x; m < n; m += blockDim.x) T[m] = tw[m];
 }
 
 // The plan's passes over nseq sequences, x -> y -> x ... (a barrier after
@@ -239,7 +246,7 @@ __device__ inline Cx* psm_transform(Cx* x, Cx* y, int nseq, const spec::Plan& p,
 template <int MaxR>
 __global__ void __launch_bounds__(spec::kThreads)
 psm_rows_fwd_kernel(const UnitDesc* __restrict__ units, int s, Cx* __restrict__ spec_all, int64_t ustride,
-                    spec::Plan px, int ntz, int np)
+                    const Cx* __restrict__ tw, spec::Plan px, int ntz, int np)
 {
     const UnitDesc& u = units[blockIdx.y];
     StepInfo si;
@@ -250,7 +257,7 @@ psm_rows_fwd_kernel(const UnitDesc* __restrict__ units, int s, Cx* __restrict__ 
     Cx* T = reinterpret_cast<Cx*>(psm_lds);
     Cx* X = T + ntx;
     Cx* Y = X + np * ntx;
-    psm_table(T, ntx);
+    psm_table(T, tw, ntx);
     const float* p1 = f.p1;
     const int pitch = u.pitch;
     spec::rows_load(X, np, ntx, ntz, pair0, [&](int iz, int ix) { return (double)p1[(int64_t)iz * pitch + ix]; },
@@ -266,17 +273,25 @@ psm_rows_fwd_kernel(const UnitDesc* __restrict__ units, int s, Cx* __restrict__ 
 template <int MaxR>
 __global__ void __launch_bounds__(spec::kThreads)
 psm_cols_kernel(const UnitDesc* __restrict__ units, int s, Cx* __restrict__ spec_all, int64_t ustride,
-                const float* __restrict__ k, spec::Plan pz, int ntx, int ncol)
+                const float* __restrict__ k, const Cx* __restrict__ tw, spec::Plan pz, int ntx, int ncol)
 {
-    const UnitDesc& u = units[blockIdx.y];
+    // Consecutive workgroups go to different XCDs (each with its own L2),
+    // and neighbouring column tiles share cache lines: give every XCD a
+    // contiguous run of tiles (a permutation of the first 8 * floor(nb / 8)
+    // workgroups; the rest keep their place).
+    const int nb = gridDim.x * gridDim.y, b = blockIdx.y * gridDim.x + blockIdx.x;
+    const int full = nb / 8 * 8;
+    const int lb = b < full ? (b % 8) * (nb / 8) + b / 8 : b;
+    const int unit = lb / gridDim.x, tile = lb - unit * gridDim.x;
+    const UnitDesc& u = units[unit];
     StepInfo si;
     if (!psm_unit_step(u, s, &si)) return;  // the whole block
-    const int ntz = pz.n, nc = ntx / 2 + 1, f0 = blockIdx.x * ncol;
-    Cx* const sp = spec_all + blockIdx.y * ustride;
+    const int ntz = pz.n, nc = ntx / 2 + 1, f0 = tile * ncol;
+    Cx* const sp = spec_all + unit * ustride;
     Cx* T = reinterpret_cast<Cx*>(psm_lds);
     Cx* X = T + ntz;
     Cx* Y = X + ncol * ntz;
-    psm_table(T, ntz);
+    psm_table(T, tw + ntx, ntz);
     spec::cols_load(X, ncol, ntz, nc, f0, [&](int iz, int fq) { return sp[(int64_t)iz * nc + fq]; }, threadIdx.x,
                     blockDim.x);
     __syncthreads();
@@ -294,7 +309,7 @@ psm_cols_kernel(const UnitDesc* __restrict__ units, int s, Cx* __restrict__ spec
 template <int MaxR>
 __global__ void __launch_bounds__(spec::kThreads)
 psm_rows_inv_kernel(const UnitDesc* __restrict__ units, int s, const Cx* __restrict__ spec_all, int64_t ustride,
-                    spec::Plan px, int ntz, int np)
+                    const Cx* __restrict__ tw, spec::Plan px, int ntz, int np)
 {
     const UnitDesc& u = units[blockIdx.y];
     StepInfo si;
@@ -304,7 +319,7 @@ psm_rows_inv_kernel(const UnitDesc* __restrict__ units, int s, const Cx* __restr
     Cx* T = reinterpret_cast<Cx*>(psm_lds);
     Cx* X = T + ntx;
     Cx* Y = X + np * ntx;
-    psm_table(T, ntx);
+    psm_table(T, tw, ntx);
     spec::rows_repack(X, np, ntx, ntz, pair0, [&](int iz, int fq) { return sp[(int64_t)iz * nc + fq]; },
                       threadIdx.x, blockDim.x);
     __syncthreads();
@@ -393,8 +408,16 @@ bool psm_spectral_supported(int ntx, int ntz)
     return psm_spectral_plans(ntx, ntz, &px, &pz, &np, &ncol);
 }
 
+hipError_t launch_psm_twiddles(hipStream_t st, double* tw, int ntx, int ntz)
+{
+    const int n = ntx + ntz;
+    hipLaunchKernelGGL(psm_twiddle_kernel, dim3((n + 255) / 256), dim3(256), 0, st, reinterpret_cast<Cx*>(tw), ntx,
+                       ntz);
+    return hipGetLastError();
+}
+
 hipError_t launch_psm_spectral(hipStream_t st, const UnitDesc* units, int nunit, int s, double* buf,
-                               const float* k, int ntx, int ntz)
+                               const float* k, const double* twd, int ntx, int ntz)
 {
     spec::Plan px, pz;
     int np, ncol;
@@ -403,6 +426,7 @@ hipError_t launch_psm_spectral(hipStream_t st, const UnitDesc* units, int nunit,
     // the buffer holds ntz * 2 nc doubles per unit = ntz * nc complex values
     const int64_t ustride = (int64_t)ntz * nc;
     Cx* sp = reinterpret_cast<Cx*>(buf);
+    const Cx* tw = reinterpret_cast<const Cx*>(twd);
     const int row_groups = ((ntz + 1) / 2 + np - 1) / np, col_groups = (nc + ncol - 1) / ncol;
     const size_t lds_rows = sizeof(Cx) * (size_t)ntx * (1 + 2 * np);
     const size_t lds_cols = sizeof(Cx) * (size_t)ntz * (1 + 2 * ncol);
@@ -410,23 +434,23 @@ hipError_t launch_psm_spectral(hipStream_t st, const UnitDesc* units, int nunit,
     const int mx = spec::max_radix(px), mz = spec::max_radix(pz);
     const dim3 rows(row_groups, nunit), cols(col_groups, nunit), blk(spec::kThreads);
     if (mx <= 7)
-        hipLaunchKernelGGL(psm_rows_fwd_kernel<7>, rows, blk, lds_rows, st, units, s, sp, ustride, px, ntz, np);
+        hipLaunchKernelGGL(psm_rows_fwd_kernel<7>, rows, blk, lds_rows, st, units, s, sp, ustride, tw, px, ntz, np);
     else if (mx == 11)
-        hipLaunchKernelGGL(psm_rows_fwd_kernel<11>, rows, blk, lds_rows, st, units, s, sp, ustride, px, ntz, np);
+        hipLaunchKernelGGL(psm_rows_fwd_kernel<11>, rows, blk, lds_rows, st, units, s, sp, ustride, tw, px, ntz, np);
     else
-        hipLaunchKernelGGL(psm_rows_fwd_kernel<13>, rows, blk, lds_rows, st, units, s, sp, ustride, px, ntz, np);
+        hipLaunchKernelGGL(psm_rows_fwd_kernel<13>, rows, blk, lds_rows, st, units, s, sp, ustride, tw, px, ntz, np);
     if (mz <= 7)
-        hipLaunchKernelGGL(psm_cols_kernel<7>, cols, blk, lds_cols, st, units, s, sp, ustride, k, pz, ntx, ncol);
+        hipLaunchKernelGGL(psm_cols_kernel<7>, cols, blk, lds_cols, st, units, s, sp, ustride, k, tw, pz, ntx, ncol);
     else if (mz == 11)
-        hipLaunchKernelGGL(psm_cols_kernel<11>, cols, blk, lds_cols, st, units, s, sp, ustride, k, pz, ntx, ncol);
+        hipLaunchKernelGGL(psm_cols_kernel<11>, cols, blk, lds_cols, st, units, s, sp, ustride, k, tw, pz, ntx, ncol);
     else
-        hipLaunchKernelGGL(psm_cols_kernel<13>, cols, blk, lds_cols, st, units, s, sp, ustride, k, pz, ntx, ncol);
+        hipLaunchKernelGGL(psm_cols_kernel<13>, cols, blk, lds_cols, st, units, s, sp, ustride, k, tw, pz, ntx, ncol);
     if (mx <= 7)
-        hipLaunchKernelGGL(psm_rows_inv_kernel<7>, rows, blk, lds_rows, st, units, s, sp, ustride, px, ntz, np);
+        hipLaunchKernelGGL(psm_rows_inv_kernel<7>, rows, blk, lds_rows, st, units, s, sp, ustride, tw, px, ntz, np);
     else if (mx == 11)
-        hipLaunchKernelGGL(psm_rows_inv_kernel<11>, rows, blk, lds_rows, st, units, s, sp, ustride, px, ntz, np);
+        hipLaunchKernelGGL(psm_rows_inv_kernel<11>, rows, blk, lds_rows, st, units, s, sp, ustride, tw, px, ntz, np);
     else
-        hipLaunchKernelGGL(psm_rows_inv_kernel<13>, rows, blk, lds_rows, st, units, s, sp, ustride, px, ntz, np);
+        hipLaunchKernelGGL(psm_rows_inv_kernel<13>, rows, blk, lds_rows, st, units, s, sp, ustride, tw, px, ntz, np);
     return hipGetLastError();
 }
 
