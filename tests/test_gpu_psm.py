@@ -153,6 +153,25 @@ def test_library_transforms(psm, monkeypatch):
     assert rel_l2(own[0], lib[0]) <= REC_RTOL and rel_l2(own[1], lib[1]) <= GRAD_RTOL
 
 
+def test_odd_lengths_and_radix_13():
+    """65 x 39 padded (5 x 13 by 3 x 13): the radix-13 kernel instantiations,
+    a last row pair holding one row, no Nyquist column, k = 3; two samples
+    in two lanes against the oracle."""
+    p, vt, vs = case("tiny", seed=11, laplace_slover=1, nx=45, nz=19, ns=2, ds=20, nr=20, dr=2, nt=240,
+                     nt_interval=3)
+    assert (p.nx + 2 * p.pml0, p.nz + 2 * p.pml0) == (65, 39)
+    obs = P.forward(p, vt)
+    syn_o, grad_o = P.fwi(p, vs, obs)
+    vel = dev(np.stack([vs.reshape(-1), vt.reshape(-1)])).view(2, -1)
+    syn, grad, mis = (host(t) for t in solver.fwi_batch(p, vel, dev(obs), ckpt_interval=0))
+    _report("65x39", syn[0], syn_o, grad[0], grad_o)
+    assert np.abs(syn_o).max() > 0
+    assert rel_l2(syn[0], syn_o) <= REC_RTOL
+    assert rel_err(mis[0], misfit64(syn_o, obs)) <= MISFIT_RTOL
+    assert rel_l2(grad[0], grad_o) <= GRAD_RTOL
+    assert rel_l2(syn[1], obs) <= REC_RTOL
+
+
 def test_length_with_a_large_prime_factor():
     """ntx = 34 = 2 x 17 has no hand-written radix: the step runs through
     hipFFT and still meets the bars."""
