@@ -398,7 +398,7 @@ def psm_leg(args, dev, p_fd, samples, obs):
            "speedup_vs_plain": round(t_plain * n / t_batch, 2),
            "us_per_time_step_batched": round(t_batch / (2 * p.nt) * 1e6, 1),
            "transforms": (f"hand-written mixed-radix in LDS, double (rows {'x'.join(map(str, rx))}, columns "
-                          f"{'x'.join(map(str, rz))}; three kernels per time step)") if own else
+                          f"{'x'.join(map(str, rz))}; two kernels per time step within a pass)") if own else
                          "hipFFT D2Z + Z2D 2-D in place, double, batch = samples x shots, one pair per time step",
            "hipfft_batched_samples_per_s": round(n / t_lib, 4),
            "hipfft_us_per_time_step_batched": round(t_lib / (2 * p.nt) * 1e6, 1)}

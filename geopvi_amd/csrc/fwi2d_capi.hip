@@ -981,18 +981,29 @@ int layout_stamp_get(const void* ws, const gpvi_fwi2d_params* p, int nsample, in
 // The pseudo-spectral steps of a batch, lane by lane: lane l advances units
 // [u0[l], u0[l] + un[l]) on stream st[l] (lanes 1.. forked from st[0] by
 // the caller) over its part of the transform buffer: with the hand-written
-// transforms when the grid allows them (three launches per step), else with
-// its own hipFFT plan pair.
+// transforms when the grid allows them (two launches per step within a
+// pass, three at its first step), else with its own hipFFT plan pair.
 int psm_run(const hipStream_t* st, const int* u0, const int* un, int nlane, const Geom& g, const Sched& sc,
             const JobDesc& j, const BatchLayout& L)
 {
     const int64_t ustride = (int64_t)g.ntz * 2 * (g.ntx / 2 + 1);   // doubles per unit in the buffer
+    // Every unit of a batch follows one schedule.  Within a pass (a run of
+    // steps of one mode: forward, a segment's recompute, an adjoint stretch)
+    // each step's p1 is the previous step's p2.
+    UnitDesc sd;
+    std::memset(&sd, 0, sizeof sd);
+    sd.nt = g.nt; sd.tb = 1; sd.nseg = sc.nseg; sd.ckpt = sc.ckpt; sd.last_len = sc.last_len;
+    auto mode_at = [&](int s) { return decode_step(sd, s + j.s_off).mode; };
     const char* lib_fft = std::getenv("GPVI_PSM_HIPFFT");  // the library transforms even where not needed (tests)
     if (psm_spectral_supported(g.ntx, g.ntz) && !(lib_fft && *lib_fft && std::strcmp(lib_fft, "0") != 0)) {
-        for (int s = 0; s < j.total; ++s)
+        for (int s = 0; s < j.total; ++s) {
+            const int mode = mode_at(s);
+            const bool first = s == 0 || mode_at(s - 1) != mode;
+            const bool chain = s + 1 < j.total && mode_at(s + 1) == mode;
             for (int l = 0; l < nlane; ++l)
                 HIP_TRY(launch_psm_spectral(st[l], L.units + u0[l], un[l], s, L.fft + u0[l] * ustride, L.ktab, L.twid,
-                                            g.ntx, g.ntz));
+                                            g.ntx, g.ntz, first, chain));
+        }
         return 0;
     }
     PsmPlans pl[kMaxLanes];
@@ -1006,16 +1017,12 @@ int psm_run(const hipStream_t* st, const int* u0, const int* un, int nlane, cons
         for (int l = 0; l < nlane; ++l)
             if ((rc = psm_plans(g.ntz, g.ntx, un[l], st[l], &pl[l]))) return rc;
     }
-    // Every unit of a batch follows one schedule.  Within a pass the point
-    // kernel leaves each step's result in the transform buffer as the next
-    // step's input; the first step of a pass (forward, a segment's
-    // recompute, an adjoint stretch) packs its p1 from the field buffers.
-    UnitDesc sd;
-    std::memset(&sd, 0, sizeof sd);
-    sd.nt = g.nt; sd.tb = 1; sd.nseg = sc.nseg; sd.ckpt = sc.ckpt; sd.last_len = sc.last_len;
+    // Within a pass the point kernel leaves each step's result in the
+    // transform buffer as the next step's input; the first step of a pass
+    // packs its p1 from the field buffers.
     int prev_mode = -1;
     for (int s = 0; s < j.total; ++s) {
-        const int mode = decode_step(sd, s + j.s_off).mode;
+        const int mode = mode_at(s);
         for (int l = 0; l < nlane; ++l) {
             const UnitDesc* units = L.units + u0[l];
             double* buf = L.fft + u0[l] * ustride;

@@ -73,11 +73,14 @@ hipError_t launch_psm_scale(hipStream_t st, double* buf, const float* k, int ntx
 hipError_t launch_psm_point(hipStream_t st, const UnitDesc* units, int nunit, int s, double* buf, int ntx,
                             int ntz);
 // The hand-written transforms: lengths ntx and ntz factor into 2, 3, 5, 7,
-// 11, 13 and fit the LDS tiling; one step = three launches.
+// 11, 13 and fit the LDS tiling.  One step: the row-forward launch when it
+// starts a pass (first), the column launch, and the row-backward launch,
+// which also does the next step's row-forward work when the next step
+// continues the pass (chain).
 bool psm_spectral_supported(int ntx, int ntz);
 hipError_t launch_psm_twiddles(hipStream_t st, double* tw, int ntx, int ntz);
 hipError_t launch_psm_spectral(hipStream_t st, const UnitDesc* units, int nunit, int s, double* buf,
-                               const float* k, const double* tw, int ntx, int ntz);
+                               const float* k, const double* tw, int ntx, int ntz, bool first, bool chain);
 hipError_t launch_misfit(hipStream_t st, const float* syn, const float* obs, int64_t len, int nsample,
                          double* partial, double* misfit);
 

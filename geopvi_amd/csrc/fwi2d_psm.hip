@@ -12,7 +12,11 @@
 //                   p2 = 2 p1 - p0 + (v dt)^2 Re / (ntx ntz), the four
 //                   ordered absorbing sweeps over the pml0-wide bands, source
 //                   or residual injection, store/records or imaging
-//                                                     (fdtd.c:819-826, 833-978)
+//                                                     (fdtd.c:819-826, 833-978);
+//                   within a pass (forward, a segment's recompute, an adjoint
+//                   stretch) also the next step's rows forward on the new
+//                   p2 rows, so rows forward runs alone only at a pass's
+//                   first step
 //
 // and otherwise (or with GPVI_PSM_HIPFFT set) through hipFFT:
 //
@@ -306,16 +310,20 @@ psm_cols_kernel(const UnitDesc* __restrict__ units, int s, Cx* __restrict__ spec
 
 // Rows backward and the point update: block (g, unit) transforms row pairs
 // [g np, (g + 1) np) of the scaled spectrum back and finishes their points.
+// chain (the next step continues this pass, so its p1 is this step's p2):
+// the block then also does the next step's rows-forward work on the p2 rows
+// it has just made, from LDS, and writes their half spectra over the ones it
+// read (each block reads and writes only its own rows).
 template <int MaxR>
 __global__ void __launch_bounds__(spec::kThreads)
-psm_rows_inv_kernel(const UnitDesc* __restrict__ units, int s, const Cx* __restrict__ spec_all, int64_t ustride,
-                    const Cx* __restrict__ tw, spec::Plan px, int ntz, int np)
+psm_rows_inv_kernel(const UnitDesc* __restrict__ units, int s, Cx* __restrict__ spec_all, int64_t ustride,
+                    const Cx* __restrict__ tw, spec::Plan px, int ntz, int np, int chain)
 {
     const UnitDesc& u = units[blockIdx.y];
     StepInfo si;
     if (!psm_unit_step(u, s, &si)) return;  // the whole block
     const int ntx = px.n, nc = ntx / 2 + 1, pair0 = blockIdx.x * np;
-    const Cx* const sp = spec_all + blockIdx.y * ustride;
+    Cx* const sp = spec_all + blockIdx.y * ustride;
     Cx* T = reinterpret_cast<Cx*>(psm_lds);
     Cx* X = T + ntx;
     Cx* Y = X + np * ntx;
@@ -324,6 +332,7 @@ psm_rows_inv_kernel(const UnitDesc* __restrict__ units, int s, const Cx* __restr
                       threadIdx.x, blockDim.x);
     __syncthreads();
     const Cx* Z = psm_transform<MaxR>(X, Y, np, px, T, -1);
+    Cx* const W = Z == X ? Y : X;   // the next step's rows (chain)
     const FieldPtrs f = step_fields(u, si);
     const int64_t margin = (int64_t)kPadRows * u.pitch;
     const StepCtx c = make_step(u, si.mode, si.it, si.seg, f.out, margin);
@@ -348,11 +357,22 @@ psm_rows_inv_kernel(const UnitDesc* __restrict__ units, int s, const Cx* __restr
             const int q = t / (2 * ntx), r = t - q * 2 * ntx;
             const int w = r >= ntx ? 1 : 0, ix = r - w * ntx;
             const int iz = 2 * (pair0 + q) + w;
-            if (t >= total || iz >= ntz) continue;
+            if (t >= total) continue;
+            double* const nxt = &W[q * ntx + ix].re + w;
+            if (iz >= ntz) {
+                if (chain) *nxt = 0.0;   // the missing row of an odd last pair
+                continue;
+            }
             const Cx z = Z[q * ntx + ix];
-            psm_point(u, f, c, kc, iz, ix, w ? z.im : z.re, ntp, core[j]);
+            const float val = psm_point(u, f, c, kc, iz, ix, w ? z.im : z.re, ntp, core[j]);
+            if (chain) *nxt = (double)val;
         }
     }
+    if (!chain) return;   // uniform over the launch
+    __syncthreads();
+    const Cx* F = psm_transform<MaxR>(W, W == X ? Y : X, np, px, T, +1);
+    spec::rows_unpack(F, np, ntx, ntz, nc, pair0, [&](int iz, int fq, Cx v) { sp[(int64_t)iz * nc + fq] = v; },
+                      threadIdx.x, blockDim.x);
 }
 
 hipError_t launch_psm_wavenumber(hipStream_t st, float* k, int ntx, int ntz, float dx, float dz)
@@ -417,7 +437,7 @@ hipError_t launch_psm_twiddles(hipStream_t st, double* tw, int ntx, int ntz)
 }
 
 hipError_t launch_psm_spectral(hipStream_t st, const UnitDesc* units, int nunit, int s, double* buf,
-                               const float* k, const double* twd, int ntx, int ntz)
+                               const float* k, const double* twd, int ntx, int ntz, bool first, bool chain)
 {
     spec::Plan px, pz;
     int np, ncol;
@@ -433,12 +453,15 @@ hipError_t launch_psm_spectral(hipStream_t st, const UnitDesc* units, int nunit,
     // instantiations by largest radix: 7, 11 or 13
     const int mx = spec::max_radix(px), mz = spec::max_radix(pz);
     const dim3 rows(row_groups, nunit), cols(col_groups, nunit), blk(spec::kThreads);
-    if (mx <= 7)
-        hipLaunchKernelGGL(psm_rows_fwd_kernel<7>, rows, blk, lds_rows, st, units, s, sp, ustride, tw, px, ntz, np);
-    else if (mx == 11)
-        hipLaunchKernelGGL(psm_rows_fwd_kernel<11>, rows, blk, lds_rows, st, units, s, sp, ustride, tw, px, ntz, np);
-    else
-        hipLaunchKernelGGL(psm_rows_fwd_kernel<13>, rows, blk, lds_rows, st, units, s, sp, ustride, tw, px, ntz, np);
+    const int ch = chain ? 1 : 0;
+    if (first) {
+        if (mx <= 7)
+            hipLaunchKernelGGL(psm_rows_fwd_kernel<7>, rows, blk, lds_rows, st, units, s, sp, ustride, tw, px, ntz, np);
+        else if (mx == 11)
+            hipLaunchKernelGGL(psm_rows_fwd_kernel<11>, rows, blk, lds_rows, st, units, s, sp, ustride, tw, px, ntz, np);
+        else
+            hipLaunchKernelGGL(psm_rows_fwd_kernel<13>, rows, blk, lds_rows, st, units, s, sp, ustride, tw, px, ntz, np);
+    }
     if (mz <= 7)
         hipLaunchKernelGGL(psm_cols_kernel<7>, cols, blk, lds_cols, st, units, s, sp, ustride, k, tw, pz, ntx, ncol);
     else if (mz == 11)
@@ -446,11 +469,13 @@ hipError_t launch_psm_spectral(hipStream_t st, const UnitDesc* units, int nunit,
     else
         hipLaunchKernelGGL(psm_cols_kernel<13>, cols, blk, lds_cols, st, units, s, sp, ustride, k, tw, pz, ntx, ncol);
     if (mx <= 7)
-        hipLaunchKernelGGL(psm_rows_inv_kernel<7>, rows, blk, lds_rows, st, units, s, sp, ustride, tw, px, ntz, np);
+        hipLaunchKernelGGL(psm_rows_inv_kernel<7>, rows, blk, lds_rows, st, units, s, sp, ustride, tw, px, ntz, np, ch);
     else if (mx == 11)
-        hipLaunchKernelGGL(psm_rows_inv_kernel<11>, rows, blk, lds_rows, st, units, s, sp, ustride, tw, px, ntz, np);
+        hipLaunchKernelGGL(psm_rows_inv_kernel<11>, rows, blk, lds_rows, st, units, s, sp, ustride, tw, px, ntz, np,
+                           ch);
     else
-        hipLaunchKernelGGL(psm_rows_inv_kernel<13>, rows, blk, lds_rows, st, units, s, sp, ustride, tw, px, ntz, np);
+        hipLaunchKernelGGL(psm_rows_inv_kernel<13>, rows, blk, lds_rows, st, units, s, sp, ustride, tw, px, ntz, np,
+                           ch);
     return hipGetLastError();
 }
 
