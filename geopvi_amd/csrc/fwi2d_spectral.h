@@ -115,48 +115,49 @@ inline int cols_per_group(int ntz)
     return c < 1 ? 0 : (c < kMaxCols ? c : kMaxCols);
 }
 
-// The length-R DFT of v in place: v[q] <- sum_r v[r] W^{rq},
-// W = e^{-sgn 2 pi i / R}.  cs[m], sn[m] = cos, sin(2 pi m / R), m = 1..(R-1)/2.
+// The length-R DFT of v, W = e^{-sgn 2 pi i / R}: d[q * ns] = sum_r v[r] W^{rq}.
+// cs[m], sn[m] = cos, sin(2 pi m / R), m = 1..(R-1)/2.  v is overwritten
+// (the odd radices form the sums and differences of the pairs r, R - r in
+// place and store each output pair as soon as it is made, so that no second
+// copy of the R values is live).
 template <int R>
-GPVI_SPEC_FN void dft(Cx* v, const double* cs, const double* sn, int sgn)
+GPVI_SPEC_FN void dft_store(Cx* v, const double* cs, const double* sn, int sgn, Cx* d, int ns)
 {
     if constexpr (R == 2) {
-        const Cx a = add(v[0], v[1]), b = sub(v[0], v[1]);
-        v[0] = a;
-        v[1] = b;
+        d[0] = add(v[0], v[1]);
+        d[ns] = sub(v[0], v[1]);
     } else if constexpr (R == 4) {
         const Cx t0 = add(v[0], v[2]), t1 = sub(v[0], v[2]);
         const Cx t2 = add(v[1], v[3]), t3 = rot(sub(v[1], v[3]), sgn);
-        v[0] = add(t0, t2);
-        v[2] = sub(t0, t2);
-        v[1] = add(t1, t3);
-        v[3] = sub(t1, t3);
+        d[0] = add(t0, t2);
+        d[2 * ns] = sub(t0, t2);
+        d[ns] = add(t1, t3);
+        d[3 * ns] = sub(t1, t3);
     } else {
         constexpr int H = (R - 1) / 2;
-        Cx s[H + 1], d[H + 1];
         Cx sum = v[0];
 #pragma unroll
         for (int r = 1; r <= H; ++r) {
-            s[r] = add(v[r], v[R - r]);
-            d[r] = sub(v[r], v[R - r]);
-            sum = add(sum, s[r]);
+            const Cx a = v[r], b = v[R - r];
+            v[r] = add(a, b);       // s_r
+            v[R - r] = sub(a, b);   // d_r
+            sum = add(sum, v[r]);
         }
-        const Cx v0 = v[0];
-        v[0] = sum;
+        d[0] = sum;
 #pragma unroll
         for (int q = 1; q <= H; ++q) {
-            Cx a = v0, b = cx(0.0, 0.0);
+            Cx a = v[0], b = cx(0.0, 0.0);
 #pragma unroll
             for (int r = 1; r <= H; ++r) {
                 const int m = (r * q) % R;
                 const double c = m <= H ? cs[m] : cs[R - m];
                 const double si = m <= H ? sn[m] : -sn[R - m];
-                a = cx(a.re + c * s[r].re, a.im + c * s[r].im);
-                b = cx(b.re + si * d[r].re, b.im + si * d[r].im);
+                a = cx(a.re + c * v[r].re, a.im + c * v[r].im);
+                b = cx(b.re + si * v[R - r].re, b.im + si * v[R - r].im);
             }
             const Cx rb = rot(b, sgn);
-            v[q] = add(a, rb);
-            v[R - q] = sub(a, rb);
+            d[q * ns] = add(a, rb);
+            d[(R - q) * ns] = sub(a, rb);
         }
     }
 }
@@ -197,10 +198,7 @@ GPVI_SPEC_FN void pass(const Cx* src, Cx* dst, int nseq, int n, int ns, const Cx
                 v[r] = mul(v[r], cx(w.re, -sgn * w.im));
             }
         }
-        dft<R>(v, cs, sn, sgn);
-        const int o = (j - k) * R + k;
-#pragma unroll
-        for (int r = 0; r < R; ++r) d[o + r * ns] = v[r];
+        dft_store<R>(v, cs, sn, sgn, d + (j - k) * R + k, ns);
     }
 }
 
