@@ -66,8 +66,9 @@ constexpr int kThreads = 256;        // workgroup of the transform kernels
 constexpr int kLdsCx = 4096;         // LDS budget of one workgroup, in Cx (64 KiB)
 // Row pairs and spectrum columns per workgroup, at most: on the 200 x 400
 // model with 10-point bands (220 x 420) one step of 4 samples x 20 shots took
-// 177 us with (2, 4), 179 with (2, 8), 206 with (2, 2), 215 with (1, 4), 232
-// with (4, 8) (256 threads; 128 threads were slower).
+// 150.7 us with (2, 4), 161.1 with (2, 8), 169.1 with (1, 4), 169.3 with
+// (2, 2), 179.2 with (4, 4), 193.0 with (1, 2) (256 threads; 128 threads were
+// slower in an earlier sweep).
 constexpr int kMaxPairs = 2;
 constexpr int kMaxCols = 4;
 
