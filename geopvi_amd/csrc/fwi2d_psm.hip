@@ -225,7 +225,7 @@ __global__ void psm_twiddle_kernel(Cx* __restrict__ tw, int ntx, int ntz)
 
 __device__ inline void psm_table(Cx* T, const Cx* __restrict__ tw, int n)
 {
-    for (int m = threadIdx.x; m < n; m += blockDim.x) T[m] = tw[m];
+    for (int m = threadIdx.x; m < spec::table_len(n); m += blockDim.x) T[m] = tw[m];
 }
 
 // The plan's passes over nseq sequences, x -> y -> x ... (a barrier after
@@ -259,7 +259,7 @@ psm_rows_fwd_kernel(const UnitDesc* __restrict__ units, int s, Cx* __restrict__ 
     const int ntx = px.n, nc = ntx / 2 + 1, pair0 = blockIdx.x * np;
     Cx* const sp = spec_all + blockIdx.y * ustride;
     Cx* T = reinterpret_cast<Cx*>(psm_lds);
-    Cx* X = T + ntx;
+    Cx* X = T + spec::table_len(ntx);
     Cx* Y = X + np * ntx;
     psm_table(T, tw, ntx);
     const float* p1 = f.p1;
@@ -293,7 +293,7 @@ psm_cols_kernel(const UnitDesc* __restrict__ units, int s, Cx* __restrict__ spec
     const int ntz = pz.n, nc = ntx / 2 + 1, f0 = tile * ncol;
     Cx* const sp = spec_all + unit * ustride;
     Cx* T = reinterpret_cast<Cx*>(psm_lds);
-    Cx* X = T + ntz;
+    Cx* X = T + spec::table_len(ntz);
     Cx* Y = X + ncol * ntz;
     psm_table(T, tw + ntx, ntz);
     spec::cols_load(X, ncol, ntz, nc, f0, [&](int iz, int fq) { return sp[(int64_t)iz * nc + fq]; }, threadIdx.x,
@@ -325,7 +325,7 @@ psm_rows_inv_kernel(const UnitDesc* __restrict__ units, int s, Cx* __restrict__ 
     const int ntx = px.n, nc = ntx / 2 + 1, pair0 = blockIdx.x * np;
     Cx* const sp = spec_all + blockIdx.y * ustride;
     Cx* T = reinterpret_cast<Cx*>(psm_lds);
-    Cx* X = T + ntx;
+    Cx* X = T + spec::table_len(ntx);
     Cx* Y = X + np * ntx;
     psm_table(T, tw, ntx);
     spec::rows_repack(X, np, ntx, ntz, pair0, [&](int iz, int fq) { return sp[(int64_t)iz * nc + fq]; },
@@ -448,8 +448,8 @@ hipError_t launch_psm_spectral(hipStream_t st, const UnitDesc* units, int nunit,
     Cx* sp = reinterpret_cast<Cx*>(buf);
     const Cx* tw = reinterpret_cast<const Cx*>(twd);
     const int row_groups = ((ntz + 1) / 2 + np - 1) / np, col_groups = (nc + ncol - 1) / ncol;
-    const size_t lds_rows = sizeof(Cx) * (size_t)ntx * (1 + 2 * np);
-    const size_t lds_cols = sizeof(Cx) * (size_t)ntz * (1 + 2 * ncol);
+    const size_t lds_rows = sizeof(Cx) * ((size_t)spec::table_len(ntx) + (size_t)ntx * 2 * np);
+    const size_t lds_cols = sizeof(Cx) * ((size_t)spec::table_len(ntz) + (size_t)ntz * 2 * ncol);
     // instantiations by largest radix: 7, 11 or 13
     const int mx = spec::max_radix(px), mz = spec::max_radix(pz);
     const dim3 rows(row_groups, nunit), cols(col_groups, nunit), blk(spec::kThreads);

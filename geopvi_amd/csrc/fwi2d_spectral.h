@@ -102,17 +102,29 @@ inline bool make_plan(int n, Plan* p)
     return m == 1;
 }
 
+// Twiddle table of a length-n transform in LDS: (cos, sin)(2 pi m / n) for
+// m < n / 2 when n is even (the second half is the first negated, exactly),
+// else for m < n.
+GPVI_SPEC_FN int table_len(int n) { return (n & 1) == 0 ? n >> 1 : n; }
+GPVI_SPEC_FN Cx twiddle(const Cx* T, int m, int n)
+{
+    const int h = (n & 1) == 0 ? n >> 1 : n;
+    const bool neg = m >= h;
+    const Cx w = T[neg ? m - h : m];
+    return neg ? cx(-w.re, -w.im) : w;
+}
+
 // Row pairs per workgroup for rows of length ntx, spectrum columns per
 // workgroup for columns of length ntz (twiddle table + two buffers within
 // the LDS budget); 0 when one does not fit.
 inline int pairs_per_group(int ntx)
 {
-    const int r = (kLdsCx - ntx) / (2 * ntx);
+    const int r = (kLdsCx - table_len(ntx)) / (2 * ntx);
     return r < 1 ? 0 : (r < kMaxPairs ? r : kMaxPairs);
 }
 inline int cols_per_group(int ntz)
 {
-    const int c = (kLdsCx - ntz) / (2 * ntz);
+    const int c = (kLdsCx - table_len(ntz)) / (2 * ntz);
     return c < 1 ? 0 : (c < kMaxCols ? c : kMaxCols);
 }
 
@@ -165,8 +177,8 @@ GPVI_SPEC_FN void dft_store(Cx* v, const double* cs, const double* sn, int sgn, 
 
 // One Stockham pass of radix R over nseq sequences of length n stored back to
 // back (src -> dst): the butterflies b = tid, tid + nthr, ... of nseq * n / R.
-// ns = product of the radices of the earlier passes.  T[m] = (cos, sin)
-// (2 pi m / n), m in [0, n).  sgn = +1: the e^{-i} transform, -1: e^{+i}.
+// ns = product of the radices of the earlier passes.  T: the twiddle table
+// (table_len, twiddle).  sgn = +1: the e^{-i} transform, -1: e^{+i}.
 template <int R>
 GPVI_SPEC_FN void pass(const Cx* src, Cx* dst, int nseq, int n, int ns, const Cx* T, int sgn, int tid, int nthr)
 {
@@ -179,7 +191,7 @@ GPVI_SPEC_FN void pass(const Cx* src, Cx* dst, int nseq, int n, int ns, const Cx
     if constexpr ((R & 1) != 0) {
 #pragma unroll
         for (int m = 1; m <= H; ++m) {
-            const Cx w = T[m * nr];
+            const Cx w = twiddle(T, m * nr, n);
             cs[m] = w.re;
             sn[m] = w.im;
         }
@@ -195,7 +207,7 @@ GPVI_SPEC_FN void pass(const Cx* src, Cx* dst, int nseq, int n, int ns, const Cx
         if (k > 0) {
 #pragma unroll
             for (int r = 1; r < R; ++r) {
-                const Cx w = T[r * k * tstep];
+                const Cx w = twiddle(T, r * k * tstep, n);
                 v[r] = mul(v[r], cx(w.re, -sgn * w.im));
             }
         }

@@ -79,7 +79,8 @@ int gpvi_fwi2d_read_params(const char *path, gpvi_fwi2d_params *out);
  * pseudo-spectral job runs as one batched call (double-precision transforms,
  * pstd_2d_calculate_p fdtd2D_modelling.c:783-830: the library's own
  * mixed-radix kernels when ntx and ntz factor into 2, 3, 5, 7, 11, 13 and
- * are at most 1365, hipFFT otherwise or with GPVI_PSM_HIPFFT=1 set). */
+ * are at most 1638 (1365 if odd), hipFFT otherwise or with GPVI_PSM_HIPFFT=1
+ * set). */
 int gpvi_fwi2d_validate(const gpvi_fwi2d_params *p);
 
 /* ---- Plain solver: one sample, shots in sequence, one launch per step ----

@@ -59,9 +59,11 @@ def test_plans(lib):
     assert plan(lib, 2 * 3 * 13 * 13) == [2, 3, 13, 13]
     for n in (17, 34, 2 * 19, 23 * 4, 0):
         assert plan(lib, n) is None
-    # LDS tiling: twiddle table + two buffers within 4096 complex values
+    # LDS tiling: twiddle table (half length for even n) + two buffers within
+    # 4096 complex values
     assert lib.spec_pairs_per_group(420) == 2 and lib.spec_cols_per_group(220) == 4
-    assert lib.spec_pairs_per_group(1365) == 1 and lib.spec_pairs_per_group(1366) == 0
+    assert lib.spec_pairs_per_group(1365) == 1 and lib.spec_pairs_per_group(1367) == 0
+    assert lib.spec_pairs_per_group(1638) == 1 and lib.spec_pairs_per_group(1640) == 0
 
 
 @pytest.mark.parametrize("n", [1, 2, 3, 4, 5, 7, 8, 11, 13, 12, 60, 64, 220, 420, 2 * 3 * 5 * 7 * 11, 13 * 13 * 2])
