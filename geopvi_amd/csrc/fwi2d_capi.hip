@@ -1582,6 +1582,25 @@ int gpvi_engine_pending_steps(gpvi_engine* e)
 // ---------------------------------------------------------------------------
 // Reference-compatible host entry points
 
+// Checkpoint interval of a one-sample call: 0 (the full wavefield store) when
+// its workspace fits in 3/4 of the free device memory, else the interval
+// that minimises a unit's memory, C* = sqrt(2 nt k ntp / ni) rounded to a
+// multiple of k (0 once it reaches nt).
+static int compat_ckpt(const gpvi_fwi2d_params* p)
+{
+    size_t free_b = 0, total_b = 0;
+    if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) free_b = 0;
+    if ((double)gpvi_fwi2d_batch_workspace_bytes(p, 1, 0) <= 0.75 * (double)free_b) return 0;
+    const int lc = p->laplace_slover == 1 ? 0 : p->Lc + 1, pml = p->pml0 + lc, k = p->nt_interval;
+    const double ntp = (double)(p->nx + 2 * pml) * (double)(p->nz + 2 * pml), ni = (double)p->nx * (double)p->nz;
+    const double c = std::sqrt(2.0 * p->nt * k * ntp / ni);
+    const int ci = std::max(k, (int)std::lround(c / k) * k);
+    return ci < p->nt ? ci : 0;
+}
+
+// One sample through the batched solver (its shots advance together), host
+// buffers in and out; fwi_2D adds to the caller's gradient, as the
+// reference does.
 static int compat_run(char input_file[200], float* vel_inner, float* record_syn, float* record_obs, float* grad,
                       int run_fwi, bool adjoint, int verbose)
 {
@@ -1592,21 +1611,28 @@ static int compat_run(char input_file[200], float* vel_inner, float* record_syn,
     if (verbose)
         std::printf("fwi2d: nx=%d nz=%d nt=%d ns=%d dx=%f dz=%f dt=%f\n", p.nx, p.nz, p.nt, p.ns, p.dx, p.dz, p.dt);
     const size_t ni = (size_t)p.nx * p.nz, nrec = (size_t)p.ns * p.nt * p.nr;
-    const size_t ws = gpvi_fwi2d_plain_workspace_bytes(&p);
+    const int ckpt = adjoint ? compat_ckpt(&p) : 0;
+    const size_t ws = adjoint ? gpvi_fwi2d_batch_workspace_bytes(&p, 1, ckpt) : gpvi_fwi2d_model_workspace_bytes(&p, 1);
     float *d_vel = nullptr, *d_syn = nullptr, *d_obs = nullptr, *d_grad = nullptr;
+    double* d_mis = nullptr;
     void* d_ws = nullptr;
     hipError_t err = hipMalloc((void**)&d_vel, ni * sizeof(float));
     if (err == hipSuccess) err = hipMalloc((void**)&d_syn, nrec * sizeof(float));
-    if (err == hipSuccess && adjoint) err = hipMalloc((void**)&d_obs, nrec * sizeof(float));
+    if (err == hipSuccess) err = hipMalloc((void**)&d_obs, nrec * sizeof(float));
     if (err == hipSuccess && adjoint) err = hipMalloc((void**)&d_grad, ni * sizeof(float));
+    if (err == hipSuccess && adjoint) err = hipMalloc((void**)&d_mis, sizeof(double));
     if (err == hipSuccess) err = hipMalloc(&d_ws, ws);
     if (err == hipSuccess) err = hipMemcpy(d_vel, vel_inner, ni * sizeof(float), hipMemcpyHostToDevice);
-    if (err == hipSuccess && adjoint) err = hipMemcpy(d_obs, record_obs, nrec * sizeof(float), hipMemcpyHostToDevice);
+    if (err == hipSuccess)
+        err = adjoint ? hipMemcpy(d_obs, record_obs, nrec * sizeof(float), hipMemcpyHostToDevice)
+                      : hipMemset(d_obs, 0, nrec * sizeof(float));
     if (err == hipSuccess && adjoint) err = hipMemcpy(d_grad, grad, ni * sizeof(float), hipMemcpyHostToDevice);
     if (err != hipSuccess) rc = fail(GPVI_ERR_HIP, "fwi_2D setup: %s", hipGetErrorString(err));
     if (!rc) {
-        rc = adjoint ? gpvi_fwi2d_fwi_device(&p, d_vel, d_syn, d_obs, d_grad, run_fwi, d_ws, ws, nullptr)
-                     : gpvi_fwi2d_forward_device(&p, d_vel, d_syn, d_ws, ws, nullptr);
+        rc = adjoint ? gpvi_fwi2d_batch(&p, 1, d_vel, d_obs, d_syn, d_grad, d_mis, run_fwi, ckpt, GPVI_PHASE_ALL, 0,
+                                        d_ws, ws, nullptr)
+                     : gpvi_fwi2d_batch(&p, 1, d_vel, d_obs, d_syn, nullptr, nullptr, 1, 0, GPVI_PHASE_FORWARD,
+                                        GPVI_FLAG_NO_STORE, d_ws, ws, nullptr);
     }
     if (!rc) {
         err = hipDeviceSynchronize();
@@ -1614,7 +1640,8 @@ static int compat_run(char input_file[200], float* vel_inner, float* record_syn,
         if (err == hipSuccess && adjoint) err = hipMemcpy(grad, d_grad, ni * sizeof(float), hipMemcpyDeviceToHost);
         if (err != hipSuccess) rc = fail(GPVI_ERR_HIP, "fwi_2D: %s", hipGetErrorString(err));
     }
-    (void)hipFree(d_vel); (void)hipFree(d_syn); (void)hipFree(d_obs); (void)hipFree(d_grad); (void)hipFree(d_ws);
+    (void)hipFree(d_vel); (void)hipFree(d_syn); (void)hipFree(d_obs); (void)hipFree(d_grad); (void)hipFree(d_mis);
+    (void)hipFree(d_ws);
     return rc;
 }
 

@@ -145,11 +145,7 @@ class Posterior:
         if self.ckpt_interval != "auto":
             ck = self.ckpt_interval
         else:
-            free, _ = torch.cuda.mem_get_info(self.device)
-            free += torch.cuda.memory_reserved(self.device) - torch.cuda.memory_allocated(self.device)
-            full = solver.batch_workspace_bytes(self.params, nsample, 0)
-            from ...engine import best_ckpt_interval
-            ck = 0 if full <= 0.75 * free else best_ckpt_interval(self.params)
+            ck = solver.auto_ckpt_interval(self.params, nsample, self.device)
         self.last_ckpt_interval = ck
         return ck
 

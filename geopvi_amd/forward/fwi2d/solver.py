@@ -154,6 +154,19 @@ def _batch_call(params, nsample, vel, obs, syn, grad, misfit, run_fwi, ckpt, pha
         int(phase), int(flags), ws.data_ptr(), ws.numel(), _stream(dev)))
 
 
+def auto_ckpt_interval(params: FWIParams, nsample: int, device) -> int:
+    """Checkpoint interval of a batched solve of ``nsample`` models: 0 (the
+    full wavefield store) when its workspace fits in 3/4 of the device
+    memory that is free or cached by torch, else the interval that minimises
+    a unit's memory (``engine.best_ckpt_interval``)."""
+    free, _ = torch.cuda.mem_get_info(device)
+    free += torch.cuda.memory_reserved(device) - torch.cuda.memory_allocated(device)
+    if batch_workspace_bytes(params, nsample, 0) <= 0.75 * free:
+        return 0
+    from ...engine import best_ckpt_interval
+    return best_ckpt_interval(params)
+
+
 def fwi_batch(params: FWIParams, vel: torch.Tensor, record_obs: torch.Tensor, run_fwi: int = 1,
               ckpt_interval: int = 0, graph: bool = True,
               workspace: Optional[torch.Tensor] = None):

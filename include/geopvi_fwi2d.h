@@ -196,8 +196,11 @@ int gpvi_engine_sync(gpvi_engine *e);
 
 /* ---- Reference-compatible host entry points ----
  * Identical signatures to pyacofwi2D.h:2-3 (fdtd2D_modelling.c:18, :356):
- * host buffers, parameter file path; they copy to the device, run the plain
- * solver and copy back.  On failure they print the message to stderr and
+ * host buffers, parameter file path; they copy to the device, run the sample
+ * as a one-sample batch (gpvi_fwi2d_batch: its shots advance together; the
+ * full wavefield store when it fits in 3/4 of the free device memory, else
+ * checkpoints every C* = sqrt(2 nt k ntp / ni) steps) and copy back.  On
+ * failure they print the message to stderr and
  * leave the outputs untouched (the reference exit()s); the status of the
  * last call on the thread is gpvi_compat_last_status(). */
 void fwi_2D(char input_file[200], float *vel_inner, float *record_syn, float *record_obs, float *grad,
