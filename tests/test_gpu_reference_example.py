@@ -10,8 +10,9 @@ z = u + exp(diag) * x as samples.  The observed data (the example's
 waveform.npy is not in the reference) are modelled from the mean plus a
 smooth seeded perturbation.
 
-* Plain (aco2d.fwi) and batched solvers on the first and last shot against
-  the oracle: records bit-exact, misfit 1e-5, gradient 1e-4.
+* The shots-in-turn solver, aco2d.fwi (a one-sample batch) and the batched
+  solver on the first and last shot against the oracle: records bit-exact,
+  misfit 1e-5, gradient 1e-4.
 * One sample over all 12 shots through the batched solver against the
   oracle, and Posterior.log_prob with the example's mask on the device
   against its host path.
@@ -79,8 +80,11 @@ def test_first_and_last_shot_plain_and_batched(example, oracle_mod, tmp_path):
     q.write(path)
     syn_o, grad_o = oracle_mod.fwi(q, samples[0], oq)
     m_o = oracle_mod.misfit_f32(syn_o, oq)
+    syn_p, grad_p = (host(t) for t in solver.fwi_plain(q, dev(samples[0]), dev(oq)))
+    print(f"shots in turn: grad rel {rel_l2(grad_p, grad_o):.2e}", flush=True)
+    assert np.array_equal(syn_p, syn_o) and rel_l2(grad_p, grad_o) <= GRAD_RTOL
     syn, grad = aco2d.fwi(samples[0].astype(np.float64), oq, paramfile=path)
-    print(f"plain: grad rel {rel_l2(grad, grad_o):.2e}", flush=True)
+    print(f"aco2d.fwi: grad rel {rel_l2(grad, grad_o):.2e}", flush=True)
     assert np.array_equal(syn, syn_o) and rel_l2(grad, grad_o) <= GRAD_RTOL
     syn_b, grad_b, mis_b = (host(t) for t in solver.fwi_batch(q, dev(samples), dev(oq)))
     print(f"batched: grad rel {rel_l2(grad_b[0], grad_o):.2e}, misfit rel {rel_err(mis_b[0], m_o):.2e}", flush=True)
