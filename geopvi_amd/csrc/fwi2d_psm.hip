@@ -205,8 +205,10 @@ __global__ void psm_point_kernel(const UnitDesc* __restrict__ units, int s, doub
 }
 
 // ---------------------------------------------------------------------------
-// Hand-written transforms (fwi2d_spectral.h): three kernels per step over a
-// per-unit half spectrum of ntz x nc complex doubles (nc = ntx/2 + 1).
+// Hand-written transforms (fwi2d_spectral.h) over a per-unit half spectrum
+// of ntz x nc complex doubles (nc = ntx/2 + 1): rows forward (first step of
+// a pass only), columns, rows backward + point update (+ the next step's
+// rows forward within a pass).
 using spec::Cx;
 
 extern __shared__ __align__(16) unsigned char psm_lds[];
