@@ -18,6 +18,7 @@ GPVI_PHASE_ADJOINT = 2
 GPVI_FLAG_NO_GRAPH = 1
 GPVI_FLAG_NO_STORE = 2
 GPVI_FLAG_ONE_STREAM = 4
+GPVI_ERR_HIP = -5  # HIP runtime failure (include/geopvi_fwi2d.h)
 GPVI_ERR_CAPACITY = -7  # engine unit/tile tables full (include/geopvi_fwi2d.h)
 
 # Every exported symbol and its ctypes signature: (restype, argtypes).

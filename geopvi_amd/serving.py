@@ -256,6 +256,9 @@ class FWIServer:
         vel: (nsample, nz*nx) models (device tensor kept as is, host arrays
         are copied); record_obs: (ns*nt*nr,) observed data on the device."""
         solver.validate(params)
+        if params.laplace_slover == 1:
+            raise ValueError("the server runs finite-difference jobs (laplace_slover=0); run pseudo-spectral ones "
+                             "through solver.fwi_batch")
         t_sub = time.perf_counter()
         if not isinstance(vel, torch.Tensor) or not vel.is_cuda:
             vel = torch.as_tensor(np.asarray(vel, np.float32)).to(self.device)
@@ -420,14 +423,15 @@ class FWIServer:
             try:
                 self.engines[lane].submit(job)
             except _lib.GpviError as exc:
-                if exc.code != _lib.GPVI_ERR_CAPACITY:
-                    raise
-                # The engine's unit slots (a contiguous run per job) or tile
-                # list are full: wait for running jobs, or fail the request
-                # when nothing runs.
+                if exc.code == _lib.GPVI_ERR_HIP:
+                    raise  # the device failed: stop, failing every waiter
                 job.workspace = None
                 self.arena.release(off, need)
-                if self._inflight:
+                # The engine's unit slots (a contiguous run per job) or tile
+                # list are full: wait for running jobs, or fail the request
+                # when nothing runs.  Any other refusal is about this request
+                # alone: fail it and go on.
+                if exc.code == _lib.GPVI_ERR_CAPACITY and self._inflight:
                     return
                 self._fail_queued(req, _lib.GpviError(exc.code, exc.msg))
                 continue

@@ -136,6 +136,8 @@ def test_guards_refuse_bad_models_and_isolate_a_nan_job(tmp_path):
         fast = torch.full_like(vel, 1.01 * p.max_stable_velocity())
         with pytest.raises(ValueError, match="stability"):
             srv.submit(p, fast, obs)
+        with pytest.raises(ValueError, match="pseudo-spectral"):   # the engine runs finite differences only
+            srv.submit(p.replace(laplace_slover=1), vel, obs)
         f1 = srv.submit(p, vel, obs)
         f_nan = srv.submit(p, vel, nan_obs)
         f2 = srv.submit(p2, vel2, obs2)
