@@ -76,6 +76,7 @@ def parse():
     ap.add_argument("--no-vi", action="store_true", help="skip the end-to-end inversion leg")
     ap.add_argument("--vi-iterations", type=int, default=2)
     ap.add_argument("--no-psm", action="store_true", help="skip the pseudo-spectral (laplace_slover=1) leg")
+    ap.add_argument("--no-fwd", action="store_true", help="skip the forward-modelling (forward_2D) leg")
     ap.add_argument("--psm-nsample", type=int, default=4)
     ap.add_argument("--seed", type=int, default=1)
     return ap.parse_args()
@@ -348,6 +349,36 @@ def _recorded_traffic():
         return json.load(f)
 
 
+def forward_leg(dev, p, vel):
+    """Forward modelling only (forward_2D, fdtd2D_modelling.c:356-675): the
+    c1 records of every sample, batched (no wavefield store), against one
+    sample through the shots-in-turn solver.  One untimed warm-up each."""
+    import torch
+    from geopvi_amd.forward.fwi2d import solver
+
+    def timed(fn):
+        fn()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        fn()
+        torch.cuda.synchronize()
+        return time.perf_counter() - t0
+
+    n = vel.shape[0]
+    t_batch = timed(lambda: solver.forward_batch(p, vel))
+    t_plain = timed(lambda: solver.forward_plain(p, vel[0].contiguous()))
+    # per unit and step: read p0, p1 and v, write p2 over the padded grid,
+    # plus the receiver row
+    ntp = (p.nx + 2 * (p.pml0 + p.Lc + 1)) * (p.nz + 2 * (p.pml0 + p.Lc + 1))
+    bytes_per_step = n * p.ns * (16 * ntp + 4 * p.nr)
+    out = {"nsample": n, "shots": p.ns, "nt": p.nt, "batched_samples_per_s": round(n / t_batch, 3),
+           "plain_samples_per_s": round(1 / t_plain, 4), "speedup_vs_plain": round(t_plain * n / t_batch, 1),
+           "us_per_time_step_batched": round(t_batch / p.nt * 1e6, 1),
+           "algorithmic_GBs": round(bytes_per_step * p.nt / t_batch / 1e9, 1)}
+    log(f"forward: batched {n / t_batch:.2f} samples/s, plain {1 / t_plain:.3f} samples/s")
+    return out
+
+
 def _radices(n):
     """The hand-written transforms' passes for length n (radix 4 first, then
     2, 3, 5, 7, 11, 13), or None when n has another prime factor."""
@@ -598,6 +629,10 @@ def main():
 
     if not args.no_psm:
         out["psm"] = psm_leg(args, dev, p, samples, obs)
+        torch.cuda.empty_cache()
+
+    if not args.no_fwd:
+        out["forward_modelling"] = forward_leg(dev, p, vel)
         torch.cuda.empty_cache()
 
     if not args.no_cpu:
