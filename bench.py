@@ -77,6 +77,8 @@ def parse():
     ap.add_argument("--vi-iterations", type=int, default=2)
     ap.add_argument("--no-psm", action="store_true", help="skip the pseudo-spectral (laplace_slover=1) leg")
     ap.add_argument("--no-fwd", action="store_true", help="skip the forward-modelling (forward_2D) leg")
+    ap.add_argument("--psm-no-hipfft", action="store_true",
+                    help="psm leg without its hipFFT comparison (for counter runs of the library's own kernels)")
     ap.add_argument("--psm-nsample", type=int, default=4)
     ap.add_argument("--seed", type=int, default=1)
     return ap.parse_args()
@@ -416,11 +418,13 @@ def psm_leg(args, dev, p_fd, samples, obs):
 
     t_batch = timed(lambda: solver.fwi_batch(p, vel, obs))
     t_plain = timed(lambda: solver.fwi_plain(p, vel[0], obs))
-    os.environ["GPVI_PSM_HIPFFT"] = "1"
-    try:
-        t_lib = timed(lambda: solver.fwi_batch(p, vel, obs))
-    finally:
-        del os.environ["GPVI_PSM_HIPFFT"]
+    t_lib = None
+    if not args.psm_no_hipfft:
+        os.environ["GPVI_PSM_HIPFFT"] = "1"
+        try:
+            t_lib = timed(lambda: solver.fwi_batch(p, vel, obs))
+        finally:
+            del os.environ["GPVI_PSM_HIPFFT"]
     ntx, ntz = p.nx + 2 * p.pml0, p.nz + 2 * p.pml0
     rx, rz = _radices(ntx), _radices(ntz)
     own = rx is not None and rz is not None
@@ -431,9 +435,10 @@ def psm_leg(args, dev, p_fd, samples, obs):
            "transforms": (f"hand-written mixed-radix in LDS, double (rows {'x'.join(map(str, rx))}, columns "
                           f"{'x'.join(map(str, rz))}; two kernels per time step within a pass)") if own else
                          "hipFFT D2Z + Z2D 2-D in place, double, batch = samples x shots, one pair per time step",
-           "hipfft_batched_samples_per_s": round(n / t_lib, 4),
-           "hipfft_us_per_time_step_batched": round(t_lib / (2 * p.nt) * 1e6, 1)}
-    log(f"psm: batched {n / t_batch:.3f} samples/s (hipFFT {n / t_lib:.3f}), plain {1 / t_plain:.3f} samples/s")
+           "hipfft_batched_samples_per_s": None if t_lib is None else round(n / t_lib, 4),
+           "hipfft_us_per_time_step_batched": None if t_lib is None else round(t_lib / (2 * p.nt) * 1e6, 1)}
+    lib_s = "not run" if t_lib is None else f"{n / t_lib:.3f}"
+    log(f"psm: batched {n / t_batch:.3f} samples/s (hipFFT {lib_s}), plain {1 / t_plain:.3f} samples/s")
     return out
 
 
