@@ -124,3 +124,14 @@ def test_unsupported_lengths_are_refused(lib):
     k = np.zeros((20, 34), np.float32)
     assert lib.spec_laplacian(p1.ctypes.data_as(ctypes.POINTER(ctypes.c_float)),
                               k.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), 20, 34, 4, 8, 64, _dp(out)) == -1
+
+
+def test_bench_reports_the_library_plans(lib):
+    """The pseudo-spectral leg of bench.py names the passes of the
+    library's plans (and hipFFT where there is none)."""
+    import importlib.util
+    spec = importlib.util.spec_from_file_location("bench_for_plans", os.path.join(os.path.dirname(HERE), "bench.py"))
+    bench = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(bench)
+    for n in range(1, 700):
+        assert bench._radices(n) == plan(lib, n), n
